@@ -1,0 +1,174 @@
+/*
+ * d2q9.h -- C ABI of the MI355X (gfx950) D2Q9 lattice Boltzmann engine.
+ *
+ * One shared library, libd2q9.so, built from pylabolt_amd/csrc/.  Plain C
+ * types only: device pointers are void*, the HIP stream is passed as void*
+ * (a hipStream_t).  Every entry point returns 0 on success, a negative
+ * D2Q9_E* code for a rejected argument, or a positive hipError_t; the text
+ * of the last error is available from d2q9_last_error().  Apart from the
+ * HIP graph wrappers, no entry point allocates memory: callers pass every
+ * buffer, including reduction workspaces.
+ *
+ * Reference interfaces replaced (file:line in Malyadeep/pylabolt):
+ *   d2q9_init_eq      <- parallel/{cpu,gpu}/equilibrium_kernels.py:38-78
+ *                        initialize_pop_density_based_second_order, bound by
+ *                        base/collision_operator.py:337-374 initialize_pop
+ *   d2q9_collide      <- parallel/{cpu,gpu}/compute_fields_kernels.py:5-65,
+ *                        force_field_kernels.py:5-25 and
+ *                        collision_kernels.py:5-153 (BGK_*), bound by
+ *                        base/compute_fields_operator.py, force_operator.py
+ *                        and collision_operator.py:289-335 collide
+ *   d2q9_stream_bc    <- parallel/{cpu,gpu}/streaming_kernels.py:5-47
+ *                        scalar_based_kernel, MPI_kernels.py:34-99 periodic
+ *                        wrap, fluid_boundary_kernels.py:5-152, bound by
+ *                        base/streaming_operator.py stream,
+ *                        parallel/MPI_operator.py:155-589 halo_exchange and
+ *                        base/boundary_operator.py:138-202 set_boundary
+ *   d2q9_fused_step   <- one whole solvers/fluidLB.py:206-253
+ *                        single_time_step (stream + boundaries of step n,
+ *                        then moments + collision of step n+1)
+ *   d2q9_moments      <- compute_fields_kernels.py:5-65 alone
+ *   d2q9_mass_fp64    <- no reference kernel; FP64 mass check
+ *   d2q9_residues     <- parallel/cpu/compute_residues_kernels.py:6-73,
+ *                        utils/residues.py:171-317
+ *   d2q9_graph_*      <- solvers/fluidLB.py:270-287 (CUDA graph of a step)
+ *   d2q9_batched_step <- no reference counterpart (many lattices per launch)
+ *   d2q9_adjoint_step <- no reference counterpart (reverse-mode step)
+ */
+#ifndef D2Q9_H
+#define D2Q9_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* storage of the populations in HBM */
+enum {
+  D2Q9_STORE_F64 = 0,     /* double, compute in double (the parity path) */
+  D2Q9_STORE_F32 = 1,     /* float, compute in float */
+  D2Q9_STORE_F16DEV = 2,  /* half of (f_i - w_i) * dev_scale_i, compute float */
+  D2Q9_STORE_BF16DEV = 3  /* bfloat16 of the same, compute float */
+};
+enum { D2Q9_BGK = 0, D2Q9_MRT = 1 };
+enum { D2Q9_FORCING_NONE = 0, D2Q9_GUO_LINEAR = 1, D2Q9_GUO_SECOND_ORDER = 2 };
+enum { D2Q9_FSRC_GRAVITY = 0, D2Q9_FSRC_FIELD = 1 };
+
+/* per-cell flags (uint8, one per padded cell) */
+enum {
+  D2Q9_CELL_FLUID = 1,      /* not solid, not ghost: updated every step */
+  D2Q9_CELL_SOLID = 2,
+  D2Q9_CELL_NEAR_SOLID = 4, /* some pull neighbour is solid */
+  D2Q9_CELL_BC = 8,         /* boundary node: bc_slot/bc_nodes valid */
+  D2Q9_CELL_STORE_MOM = 16  /* rho/u read by a neighbour next step */
+};
+
+/* boundary rule of one direction of one node: kind | (element << 4) */
+enum {
+  D2Q9_RULE_STREAM = 0,   /* keep the streamed value */
+  D2Q9_RULE_BB = 1,       /* f[inv] = f*[out] */
+  D2Q9_RULE_VEL = 2,      /* moving wall: f*[out] - 2 w rho cs^-2 c.U */
+  D2Q9_RULE_PRES = 3,     /* anti-bounce-back at fixed density */
+  D2Q9_RULE_ZG = 4        /* copy from the inward neighbour, streamed */
+};
+
+/* error codes (negative) */
+enum {
+  D2Q9_E_ARG = -1,        /* null pointer, bad enum or size */
+  D2Q9_E_TOO_LARGE = -2,  /* workspace too small */
+  D2Q9_E_UNSUPPORTED = -3 /* combination not implemented */
+};
+
+typedef struct d2q9_bc_elem {
+  double vec[2];    /* fixed_velocity wall velocity */
+  double scalar;    /* fixed_pressure density */
+  int32_t normal[2];/* inward normal */
+  int32_t type;     /* D2Q9_RULE_* of this element */
+  int32_t pad;
+} d2q9_bc_elem;
+
+typedef struct d2q9_bc_node {
+  uint16_t rule[9];
+  uint16_t pad;
+} d2q9_bc_node;
+
+/*
+ * One lattice.  Padded cell (i, j), 0 <= i <= nx+1, 0 <= j <= ny+1, lives at
+ * element origin + i * pitch + j of every plane; plane q of the populations
+ * starts at q * plane.  The engine chooses origin = 63 and pitch a multiple
+ * of 64 so that every interior row starts on a 64-element boundary.
+ */
+typedef struct d2q9_lattice {
+  int32_t nx, ny, pitch, origin;
+  int64_t plane;
+  int32_t storage, collision, forcing, force_source;
+  int32_t x_periodic, y_periodic;
+  int32_t tile_j, tile_i;       /* launch tile: tile_j * tile_i == 256 */
+  double omega, inv_cs_2, inv_cs_4, float_min;
+  double w[9];
+  double gravity[2];
+  double rates[9];              /* MRT relaxation rates */
+  double dev_scale[9];          /* 16-bit storage scale per direction */
+  const uint8_t* flags;         /* [plane] */
+  const uint8_t* tile_class;    /* [tiles]: 0 = plain tile, or NULL */
+  const int32_t* bc_slot;       /* [plane], read where D2Q9_CELL_BC */
+  const d2q9_bc_node* bc_nodes;
+  const d2q9_bc_elem* bc_elems;
+  const void* force_field;      /* [2][plane] when force_source == FIELD */
+} d2q9_lattice;
+
+/* Populations f = feq(rho0, u0) on fluid cells of both buffers, the encoded
+ * zero elsewhere.  mom = [rho | ux | uy] planes of the initial fields. */
+int d2q9_init_eq(const d2q9_lattice* L, void* f_a, void* f_b, const void* mom,
+                 void* stream);
+
+/* Moments + collision only (the first step's pre-collision state is the
+ * initial equilibrium): f_dst = C(f_src).  Writes rho/u into mom_dst where
+ * D2Q9_CELL_STORE_MOM, or on every fluid cell if write_moments. */
+int d2q9_collide(const d2q9_lattice* L, const void* f_src, void* f_dst,
+                 const void* mom_src, void* mom_dst, int write_moments,
+                 void* stream);
+
+/* Stream + periodic wrap + boundary rules only: f_dst = pop_new. */
+int d2q9_stream_bc(const d2q9_lattice* L, const void* f_src, void* f_dst,
+                   const void* mom_src, void* stream);
+
+/* Fused: stream + boundaries of step n, moments + collision of step n+1. */
+int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
+                    const void* mom_src, void* mom_dst, int write_moments,
+                    void* stream);
+
+/* rho/u of the populations f (e.g. pop_new) on every fluid cell. */
+int d2q9_moments(const d2q9_lattice* L, const void* f, void* mom,
+                 void* stream);
+
+/* FP64 sum of all populations over fluid cells into *out (device double).
+ * work: device doubles, at least d2q9_reduce_work_len(L) of them. */
+int64_t d2q9_reduce_work_len(const d2q9_lattice* L);
+int d2q9_mass_fp64(const d2q9_lattice* L, const void* f, double* work,
+                   double* out, void* stream);
+
+/* Residue sums over fluid cells of mom vs mom_old (FP64):
+ * out[0..5] = sum drho^2, sum rho_old^2, sum dux^2, sum ux_old^2,
+ *             sum duy^2, sum uy_old^2; then mom_old = mom. */
+int d2q9_residues(const d2q9_lattice* L, const void* mom, void* mom_old,
+                  double* work, double* out, void* stream);
+
+/* Plain copy kernel (bandwidth reference). */
+int d2q9_copy(void* dst, const void* src, int64_t bytes, void* stream);
+
+/* HIP graph capture of the work issued on `stream` between begin and end. */
+int d2q9_graph_begin(void* stream);
+int d2q9_graph_end(void* stream, void** graph_exec);
+int d2q9_graph_launch(void* graph_exec, void* stream);
+int d2q9_graph_destroy(void* graph_exec);
+
+/* Text of the last error on this thread. */
+const char* d2q9_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* D2Q9_H */

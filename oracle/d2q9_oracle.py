@@ -1,0 +1,417 @@
+"""FP64 CPU restatement of the reference fluidLB D2Q9 time step.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP engine in
+``pylabolt_amd``: only ``tests/``, ``__graft_entry__.smoke()`` and the CPU
+baseline leg of ``bench.py`` may import it.  The product never calls it.
+
+It restates, pass by pass and in the reference's operation order, the
+multi-pass step of ``pylabolt/solvers/fluidLB.py:206-253`` over the
+reference's AoS layout ``pop[size, 9]`` with the padded index
+``ind = i * Ny_pad + j``:
+
+1. density   parallel/cpu/compute_fields_kernels.py:5-28
+2. force     parallel/cpu/force_field_kernels.py:5-25
+3. velocity  parallel/cpu/compute_fields_kernels.py:31-65
+4. collide   parallel/cpu/collision_kernels.py:5-153 (BGK x 3 forcings)
+5. periodic  parallel/gpu/MPI_kernels.py:34-99 (single-rank wrap of the
+             post-collision populations, x columns first, then y rows)
+6. stream    parallel/cpu/streaming_kernels.py:5-47
+7. boundary  parallel/cpu/fluid_boundary_kernels.py:5-62,100-152, one
+             element after another in dict/segment order
+
+NumPy applies each elementwise operation with one IEEE rounding, so writing
+every expression with the reference's association (Python evaluates left to
+right) gives the same bits the reference's scalar loops give without
+fast-math.
+
+Parts with no counterpart in the reference are defined here and in the HIP
+engine identically (parity for them is against this definition):
+
+* MRT: ``f* = f - M^-1 [ S (M f - M feq) - (I - S/2) M Fbar ]`` with the
+  reference's moment matrix M and ``S = (1,1,1,1,1,1,1,omega,omega)``
+  (base/collision_operator.py:136-165, whose own setup is broken); Fbar is
+  the Guo source of the selected forcing without its (1 - omega/2) factor.
+  The operation order is spelled out in ``mrt_collide``.
+* zero_gradient: the unknown directions of a boundary node take the values
+  the streaming pass produced in the same directions at the inward
+  neighbour (before any boundary element ran).
+* force_source "field": the force field is a fixed per-cell input instead
+  of ``rho * gravity``.
+* fixed_pressure uses the integer inward normal (the reference indexes with
+  the float normal, fluid_boundary_kernels.py:133-135).
+
+The reference's CPU single-rank halo exchange sends and receives with its
+own rank (MPI_operator.py:261-369), which copies column 1 into ghost column
+0 instead of wrapping; the single-GPU path and the multi-rank path wrap
+periodically, and so does this restatement.
+"""
+import numpy as np
+
+CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1], dtype=np.int64)
+CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1], dtype=np.int64)
+INV = np.array([0, 3, 4, 1, 2, 7, 8, 5, 6], dtype=np.int64)
+
+# Reference moment matrix (base/collision_operator.py:147-157).
+M_MRT = np.array([
+    [1, 1, 1, 1, 1, 1, 1, 1, 1],
+    [-4, -1, -1, -1, -1, 2, 2, 2, 2],
+    [4, -2, -2, -2, -2, 1, 1, 1, 1],
+    [0, 1, 0, -1, 0, 1, -1, -1, 1],
+    [0, -2, 0, 2, 0, 1, -1, -1, 1],
+    [0, 0, 1, 0, -1, 1, 1, -1, -1],
+    [0, 0, -2, 0, 2, 1, 1, -1, -1],
+    [0, 1, -1, 1, -1, 0, 0, 0, 0],
+    [0, 0, 0, 0, 0, 1, -1, 1, -1],
+], dtype=np.int64)
+# The rows of M are orthogonal: M^-1 = M^T diag(1 / |row|^2).
+MRT_ROW_NORM = np.array([9, 36, 36, 6, 12, 6, 12, 4, 4], dtype=np.int64)
+
+
+def lattice_constants():
+    """Constants computed as base/lattice.py:41-59, base/control.py:49."""
+    cs = np.float64(1 / np.sqrt(3))
+    cs_2 = cs * cs
+    inv_cs_2 = 1.0 / cs_2
+    inv_cs_4 = inv_cs_2 * inv_cs_2
+    w = np.array([4/9, 1/9, 1/9, 1/9, 1/9, 1/36, 1/36, 1/36, 1/36],
+                 dtype=np.float64)
+    float_min = np.finfo(np.float64).eps
+    return dict(cs=cs, cs_2=cs_2, inv_cs_2=np.float64(inv_cs_2),
+                inv_cs_4=np.float64(inv_cs_4), w=w,
+                float_min=np.float64(float_min))
+
+
+def omega_from_visc(kin_visc):
+    """base/collision_operator.py:89-91."""
+    c = lattice_constants()
+    tau = np.float64(kin_visc) * c["inv_cs_2"] + 0.5
+    return np.float64(1 / tau)
+
+
+class OracleCase:
+    """Everything one run needs, as plain arrays.
+
+    ``elements`` is the ordered list of boundary elements; each is a dict
+    with keys ``type`` ("bounce_back", "fixed_velocity", "fixed_pressure",
+    "zero_gradient", "periodic"), ``nodes`` (int array of flat indices),
+    ``out``/``inv`` (direction lists), ``vector`` (2 floats), ``scalar``
+    (float) and ``normal`` (2 ints).
+    """
+
+    def __init__(self, shape, solid, ghost, density, velocity, elements,
+                 x_periodic, y_periodic, omega, model="BGK", forcing=None,
+                 gravity=(0.0, 0.0), force_source="gravity",
+                 force_field=None):
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.size = self.shape[0] * self.shape[1]
+        self.solid = np.asarray(solid, dtype=bool).copy()
+        self.ghost = np.asarray(ghost, dtype=bool).copy()
+        self.fluid = ~self.solid & ~self.ghost
+        self.density = np.asarray(density, dtype=np.float64).copy()
+        self.velocity = np.asarray(velocity, dtype=np.float64).copy()
+        self.elements = elements
+        self.x_periodic = bool(x_periodic)
+        self.y_periodic = bool(y_periodic)
+        self.omega = np.float64(omega)
+        self.model = model
+        self.forcing = forcing
+        self.gravity = np.asarray(gravity, dtype=np.float64)
+        self.force_source = force_source
+        self.force_field = np.zeros((self.size, 2), dtype=np.float64)
+        if force_field is not None:
+            self.force_field[:] = force_field
+        self.pop = np.zeros((self.size, 9), dtype=np.float64)
+        self.pop_new = np.zeros((self.size, 9), dtype=np.float64)
+        self.const = lattice_constants()
+        self.time_step = 0
+
+
+def equilibrium(rho, ux, uy, k, c):
+    """feq_k (parallel/cpu/equilibrium_kernels.py:6-35), vectorised."""
+    inv_cs_2, inv_cs_4 = c["inv_cs_2"], c["inv_cs_4"]
+    u2 = ux * ux + uy * uy
+    cu = CX[k] * ux + CY[k] * uy
+    return c["w"][k] * rho * (
+        1 + inv_cs_2 * cu + 0.5 * inv_cs_4 * cu * cu - 0.5 * inv_cs_2 * u2
+    )
+
+
+def initialize_pop(case):
+    """pop = pop_new = feq(rho0, u0) on fluid nodes
+    (parallel/cpu/equilibrium_kernels.py:38-78)."""
+    m = case.fluid
+    rho = case.density[m]
+    ux, uy = case.velocity[m, 0], case.velocity[m, 1]
+    for k in range(9):
+        feq = equilibrium(rho, ux, uy, k, case.const)
+        case.pop[m, k] = feq
+        case.pop_new[m, k] = feq
+
+
+def compute_density(case):
+    m = case.fluid
+    s = np.zeros(int(m.sum()), dtype=np.float64)
+    pn = case.pop_new[m]
+    for k in range(9):
+        s += pn[:, k]
+    case.density[m] = s
+
+
+def compute_force(case):
+    if case.force_source != "gravity":
+        return
+    m = case.fluid
+    rho = case.density[m]
+    case.force_field[m, 0] = rho * case.gravity[0]
+    case.force_field[m, 1] = rho * case.gravity[1]
+
+
+def compute_velocity(case):
+    m = case.fluid
+    pn = case.pop_new[m]
+    sx = np.zeros(pn.shape[0], dtype=np.float64)
+    sy = np.zeros(pn.shape[0], dtype=np.float64)
+    for k in range(9):
+        sx += CX[k] * pn[:, k]
+        sy += CY[k] * pn[:, k]
+    inv = 1 / (case.density[m] + case.const["float_min"])
+    fx = case.force_field[m, 0]
+    fy = case.force_field[m, 1]
+    case.velocity[m, 0] = sx * inv + 0.5 * fx * inv
+    case.velocity[m, 1] = sy * inv + 0.5 * fy * inv
+
+
+def guo_source(k, ux, uy, fx, fy, forcing, c):
+    """Guo source term without the (1 - omega/2) factor
+    (collision_kernels.py:85-87 and 138-144)."""
+    inv_cs_2, inv_cs_4 = c["inv_cs_2"], c["inv_cs_4"]
+    if forcing == "guo_linear":
+        return c["w"][k] * (CX[k] * fx + CY[k] * fy) * inv_cs_2
+    cu = CX[k] * ux + CY[k] * uy
+    const_x = (CX[k] - ux) * inv_cs_2 + cu * CX[k] * inv_cs_4
+    const_y = (CY[k] - uy) * inv_cs_2 + cu * CY[k] * inv_cs_4
+    return c["w"][k] * (const_x * fx + const_y * fy)
+
+
+def bgk_collide(case):
+    m = case.fluid
+    c = case.const
+    om = case.omega
+    rho = case.density[m]
+    ux, uy = case.velocity[m, 0], case.velocity[m, 1]
+    fx, fy = case.force_field[m, 0], case.force_field[m, 1]
+    pn = case.pop_new[m]
+    out = np.empty_like(pn)
+    for k in range(9):
+        feq = equilibrium(rho, ux, uy, k, c)
+        if case.forcing is None:
+            out[:, k] = (1 - om) * pn[:, k] + om * feq
+        else:
+            ft = guo_source(k, ux, uy, fx, fy, case.forcing, c)
+            out[:, k] = (1 - om) * pn[:, k] + om * feq + (1 - 0.5 * om) * ft
+    case.pop[m] = out
+
+
+def mrt_moments(vals):
+    """m_i = sum_k M[i,k] * v_k, accumulated in k order from 0.0."""
+    n = vals.shape[0]
+    m = np.zeros((n, 9), dtype=np.float64)
+    for i in range(9):
+        acc = np.zeros(n, dtype=np.float64)
+        for k in range(9):
+            if M_MRT[i, k] != 0:
+                acc = acc + np.float64(M_MRT[i, k]) * vals[:, k]
+        m[:, i] = acc
+    return m
+
+
+def mrt_collide(case, rates=None):
+    """Build-defined MRT with Guo forcing (see module docstring).
+
+    With ``s = rates``:
+      m  = M f, meq = M feq, mF = M Fbar          (mrt_moments order)
+      d_i = s_i * (m_i - meq_i) - (1 - 0.5 * s_i) * mF_i
+      g_i = d_i * (1 / |row_i|^2)
+      f*_k = f_k - (sum over i in order of M[i,k] * g_i, from 0.0)
+    """
+    m_ = case.fluid
+    c = case.const
+    om = case.omega
+    if rates is None:
+        rates = np.array([1, 1, 1, 1, 1, 1, 1, om, om], dtype=np.float64)
+    rho = case.density[m_]
+    ux, uy = case.velocity[m_, 0], case.velocity[m_, 1]
+    fx, fy = case.force_field[m_, 0], case.force_field[m_, 1]
+    pn = case.pop_new[m_]
+    n = pn.shape[0]
+    feq = np.empty((n, 9))
+    fbar = np.zeros((n, 9))
+    for k in range(9):
+        feq[:, k] = equilibrium(rho, ux, uy, k, c)
+        if case.forcing is not None:
+            fbar[:, k] = guo_source(k, ux, uy, fx, fy, case.forcing, c)
+    mm = mrt_moments(pn)
+    meq = mrt_moments(feq)
+    mf = mrt_moments(fbar)
+    g = np.empty((n, 9))
+    for i in range(9):
+        s = np.float64(rates[i])
+        d = s * (mm[:, i] - meq[:, i]) - (1 - 0.5 * s) * mf[:, i]
+        g[:, i] = d * np.float64(1.0 / MRT_ROW_NORM[i])
+    out = np.empty_like(pn)
+    for k in range(9):
+        acc = np.zeros(n)
+        for i in range(9):
+            if M_MRT[i, k] != 0:
+                acc = acc + np.float64(M_MRT[i, k]) * g[:, i]
+        out[:, k] = pn[:, k] - acc
+    case.pop[m_] = out
+
+
+def periodic_wrap(case):
+    """Single-rank wrap of post-collision populations
+    (parallel/gpu/MPI_kernels.py:34-99): columns over all j, then rows over
+    all i, so ghost corners receive the diagonal image."""
+    nx, ny = case.shape
+    p = case.pop.reshape(nx, ny, 9)
+    if case.x_periodic:
+        left = p[nx - 2, :, :].copy()
+        right = p[1, :, :].copy()
+        p[0, :, :] = left
+        p[nx - 1, :, :] = right
+    if case.y_periodic:
+        bottom = p[:, ny - 2, :].copy()
+        top = p[:, 1, :].copy()
+        p[:, 0, :] = bottom
+        p[:, ny - 1, :] = top
+
+
+def stream(case):
+    """Pull streaming with moving-wall bounce-back at solid neighbours
+    (parallel/cpu/streaming_kernels.py:5-47)."""
+    c = case.const
+    nx, ny = case.shape
+    idx = np.flatnonzero(case.fluid)
+    x = idx // ny
+    y = idx - x * ny
+    rho = case.density[idx]
+    case.pop_new[idx, 0] = case.pop[idx, 0]
+    for k in range(1, 9):
+        nb = (x - CX[k]) * ny + (y - CY[k])
+        sol = case.solid[nb]
+        val = case.pop[nb, k].copy()
+        if np.any(sol):
+            ki = INV[k]
+            s_idx = idx[sol]
+            temp = 2 * c["w"][ki] * rho[sol] * c["inv_cs_2"] * (
+                CX[ki] * case.velocity[nb[sol], 0] +
+                CY[ki] * case.velocity[nb[sol], 1]
+            )
+            val[sol] = case.pop[s_idx, ki] - temp
+        case.pop_new[idx, k] = val
+
+
+def apply_boundaries(case, streamed):
+    """Boundary elements in order (fluid_boundary_kernels.py:5-152).
+    ``streamed`` is pop_new as the streaming pass left it (used by
+    zero_gradient only)."""
+    c = case.const
+    nx, ny = case.shape
+    for e in case.elements:
+        t = e["type"]
+        if t == "periodic":
+            continue
+        nodes = np.asarray(e["nodes"], dtype=np.int64)
+        if nodes.size == 0:
+            continue
+        nodes = nodes[~case.solid[nodes]]
+        outl, invl = e["out"], e["inv"]
+        if t == "bounce_back":
+            for o, v in zip(outl, invl):
+                case.pop_new[nodes, v] = case.pop[nodes, o]
+        elif t == "fixed_velocity":
+            uw = np.asarray(e["vector"], dtype=np.float64)
+            rho = case.density[nodes]
+            for o, v in zip(outl, invl):
+                temp = 2 * c["w"][o] * rho * c["inv_cs_2"] * (
+                    CX[o] * uw[0] + CY[o] * uw[1]
+                )
+                case.pop_new[nodes, v] = case.pop[nodes, o] - temp
+        elif t == "fixed_pressure":
+            rho_b = np.float64(e["scalar"])
+            nvec = e["normal"]
+            nb = nodes + int(nvec[0]) * ny + int(nvec[1])
+            vx = case.velocity[nodes, 0]
+            vy = case.velocity[nodes, 1]
+            ex = vx + 0.5 * (vx - case.velocity[nb, 0])
+            ey = vy + 0.5 * (vy - case.velocity[nb, 1])
+            u2 = ex * ex + ey * ey
+            for o, v in zip(outl, invl):
+                cu = CX[o] * ex + CY[o] * ey
+                temp = 2 * c["w"][o] * rho_b * (
+                    1 + 0.5 * c["inv_cs_4"] * cu * cu -
+                    0.5 * c["inv_cs_2"] * u2
+                )
+                case.pop_new[nodes, v] = -case.pop[nodes, o] + temp
+        elif t == "zero_gradient":
+            nvec = e["normal"]
+            nb = nodes + int(nvec[0]) * ny + int(nvec[1])
+            for v in invl:
+                case.pop_new[nodes, v] = streamed[nb, v]
+        else:
+            raise ValueError("unknown boundary type " + str(t))
+
+
+def single_time_step(case):
+    """One reference time step (solvers/fluidLB.py:206-253)."""
+    compute_density(case)
+    compute_force(case)
+    compute_velocity(case)
+    if case.model == "BGK":
+        bgk_collide(case)
+    elif case.model == "MRT":
+        mrt_collide(case)
+    else:
+        raise ValueError("unknown collision model " + str(case.model))
+    periodic_wrap(case)
+    stream(case)
+    needs_snapshot = any(e["type"] == "zero_gradient" for e in case.elements)
+    streamed = case.pop_new.copy() if needs_snapshot else None
+    apply_boundaries(case, streamed)
+    case.time_step += 1
+
+
+def run(case, steps):
+    for _ in range(steps):
+        single_time_step(case)
+    return case
+
+
+def case_from_state(state, model, forcing, omega, gravity,
+                    force_source="gravity", force_field=None):
+    """Build an ``OracleCase`` from a case-state object.
+
+    ``state`` is duck-typed: it needs ``domain.shape``, ``fields.solid``,
+    ``fields.ghost_node``, ``fields.density``, ``fields.velocity`` and
+    ``boundary.{boundary_elements, x_periodic, y_periodic}`` with element
+    attributes ``type_fluid, boundary_nodes, out_list, inv_list,
+    vector_fluid, scalar_fluid`` and ``surface_normals`` -- the reference's
+    names (base/boundary.py:7-84).
+    """
+    elements = []
+    for e in state.boundary.boundary_elements:
+        elements.append(dict(
+            type=e.type_fluid,
+            nodes=np.asarray(e.boundary_nodes, dtype=np.int64),
+            out=[int(v) for v in e.out_list],
+            inv=[int(v) for v in e.inv_list],
+            vector=[float(v) for v in e.vector_fluid],
+            scalar=float(e.scalar_fluid),
+            normal=[int(round(float(v))) for v in e.surface_normals],
+        ))
+    f = state.fields
+    return OracleCase(state.domain.shape, f.solid, f.ghost_node, f.density,
+                      f.velocity, elements, state.boundary.x_periodic,
+                      state.boundary.y_periodic, omega, model=model,
+                      forcing=forcing, gravity=gravity,
+                      force_source=force_source, force_field=force_field)

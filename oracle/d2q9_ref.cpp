@@ -1,0 +1,298 @@
+// FP64 CPU restatement of the reference fluidLB D2Q9 time step, C++/OpenMP.
+//
+// TEST INFRASTRUCTURE ONLY: the checker and the timed CPU baseline.  Only the
+// test suite, the smoke check and the CPU-baseline leg of the benchmark load
+// it; the product never does.
+//
+// Same passes, same AoS layout pop[size][9] with ind = i * Ny_pad + j, and the
+// same operation order as oracle/d2q9_oracle.py, which documents each pass
+// against the reference files (pylabolt/solvers/fluidLB.py:206-253 and the
+// kernels in pylabolt/parallel/cpu/).  Built with -ffp-contract=off so that
+// no multiply-add is fused: the results equal the NumPy restatement bit for
+// bit.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+namespace {
+const int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+const int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+const int INV[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+const int MM[9][9] = {
+    {1, 1, 1, 1, 1, 1, 1, 1, 1},   {-4, -1, -1, -1, -1, 2, 2, 2, 2},
+    {4, -2, -2, -2, -2, 1, 1, 1, 1}, {0, 1, 0, -1, 0, 1, -1, -1, 1},
+    {0, -2, 0, 2, 0, 1, -1, -1, 1}, {0, 0, 1, 0, -1, 1, 1, -1, -1},
+    {0, 0, -2, 0, 2, 1, 1, -1, -1}, {0, 1, -1, 1, -1, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 1, -1, 1, -1}};
+const int ROWNORM[9] = {9, 36, 36, 6, 12, 6, 12, 4, 4};
+}  // namespace
+
+extern "C" {
+
+// Boundary element kinds, in the order the Python side encodes them.
+enum { OR_BB = 0, OR_VEL = 1, OR_PRES = 2, OR_ZG = 3, OR_PERIODIC = 4 };
+
+typedef struct oracle_elem {
+  int type;
+  int n_nodes;
+  const int64_t* nodes;
+  int out[3];
+  int inv[3];
+  double vec[2];
+  double scalar;
+  int normal[2];
+} oracle_elem;
+
+typedef struct oracle_case {
+  int nx_pad, ny_pad;
+  const uint8_t* solid;
+  const uint8_t* ghost;
+  double* density;   // [size]
+  double* velocity;  // [size][2]
+  double* force;     // [size][2]
+  double* pop;       // [size][9]
+  double* pop_new;   // [size][9]
+  double omega;
+  double inv_cs_2, inv_cs_4, float_min;
+  double w[9];
+  int model;         // 0 BGK, 1 MRT
+  int forcing;       // 0 none, 1 guo_linear, 2 guo_second_order
+  int force_source;  // 0 gravity, 1 field
+  double gravity[2];
+  int x_periodic, y_periodic;
+  int n_elements;
+  const oracle_elem* elems;
+} oracle_case;
+
+static inline double feq_k(const oracle_case* c, int k, double rho, double ux,
+                           double uy) {
+  double u2 = ux * ux + uy * uy;
+  double cu = (double)CX[k] * ux + (double)CY[k] * uy;
+  return c->w[k] * rho *
+         (((1.0 + c->inv_cs_2 * cu) + ((0.5 * c->inv_cs_4) * cu) * cu) -
+          (0.5 * c->inv_cs_2) * u2);
+}
+
+static inline double guo_k(const oracle_case* c, int k, double ux, double uy,
+                           double fx, double fy) {
+  if (c->forcing == 1)
+    return (c->w[k] * ((double)CX[k] * fx + (double)CY[k] * fy)) * c->inv_cs_2;
+  double cu = (double)CX[k] * ux + (double)CY[k] * uy;
+  double const_x = ((double)CX[k] - ux) * c->inv_cs_2 + (cu * (double)CX[k]) * c->inv_cs_4;
+  double const_y = ((double)CY[k] - uy) * c->inv_cs_2 + (cu * (double)CY[k]) * c->inv_cs_4;
+  return c->w[k] * (const_x * fx + const_y * fy);
+}
+
+int oracle_initialize(oracle_case* c) {
+  const long size = (long)c->nx_pad * c->ny_pad;
+#pragma omp parallel for schedule(static)
+  for (long ind = 0; ind < size; ++ind) {
+    if (c->solid[ind] || c->ghost[ind]) continue;
+    double rho = c->density[ind];
+    double ux = c->velocity[2 * ind], uy = c->velocity[2 * ind + 1];
+    for (int k = 0; k < 9; ++k) {
+      double f = feq_k(c, k, rho, ux, uy);
+      c->pop[9 * ind + k] = f;
+      c->pop_new[9 * ind + k] = f;
+    }
+  }
+  return 0;
+}
+
+static void moments_and_collide(oracle_case* c) {
+  const long size = (long)c->nx_pad * c->ny_pad;
+  const double om = c->omega;
+  // density, force, velocity: three passes as in the reference.
+#pragma omp parallel for schedule(static)
+  for (long ind = 0; ind < size; ++ind) {
+    if (c->solid[ind] || c->ghost[ind]) continue;
+    double s = 0.0;
+    for (int k = 0; k < 9; ++k) s += c->pop_new[9 * ind + k];
+    c->density[ind] = s;
+  }
+  if (c->force_source == 0) {
+#pragma omp parallel for schedule(static)
+    for (long ind = 0; ind < size; ++ind) {
+      if (c->solid[ind] || c->ghost[ind]) continue;
+      double rho = c->density[ind];
+      c->force[2 * ind] = rho * c->gravity[0];
+      c->force[2 * ind + 1] = rho * c->gravity[1];
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (long ind = 0; ind < size; ++ind) {
+    if (c->solid[ind] || c->ghost[ind]) continue;
+    double sx = 0.0, sy = 0.0;
+    const double* pn = c->pop_new + 9 * ind;
+    for (int k = 0; k < 9; ++k) {
+      sx += (double)CX[k] * pn[k];
+      sy += (double)CY[k] * pn[k];
+    }
+    double inv = 1.0 / (c->density[ind] + c->float_min);
+    c->velocity[2 * ind] = sx * inv + (0.5 * c->force[2 * ind]) * inv;
+    c->velocity[2 * ind + 1] = sy * inv + (0.5 * c->force[2 * ind + 1]) * inv;
+  }
+#pragma omp parallel for schedule(static)
+  for (long ind = 0; ind < size; ++ind) {
+    if (c->solid[ind] || c->ghost[ind]) continue;
+    double rho = c->density[ind];
+    double ux = c->velocity[2 * ind], uy = c->velocity[2 * ind + 1];
+    double fx = c->force[2 * ind], fy = c->force[2 * ind + 1];
+    const double* pn = c->pop_new + 9 * ind;
+    double* po = c->pop + 9 * ind;
+    if (c->model == 0) {
+      for (int k = 0; k < 9; ++k) {
+        double feq = feq_k(c, k, rho, ux, uy);
+        double v = (1.0 - om) * pn[k] + om * feq;
+        if (c->forcing != 0) v = v + (1.0 - 0.5 * om) * guo_k(c, k, ux, uy, fx, fy);
+        po[k] = v;
+      }
+    } else {
+      double feq[9], fb[9], mm[9], me[9], mf[9], g[9];
+      for (int k = 0; k < 9; ++k) {
+        feq[k] = feq_k(c, k, rho, ux, uy);
+        fb[k] = c->forcing ? guo_k(c, k, ux, uy, fx, fy) : 0.0;
+      }
+      for (int i = 0; i < 9; ++i) {
+        double a = 0.0, b = 0.0, d = 0.0;
+        for (int k = 0; k < 9; ++k) {
+          if (MM[i][k] == 0) continue;
+          a = a + (double)MM[i][k] * pn[k];
+          b = b + (double)MM[i][k] * feq[k];
+          d = d + (double)MM[i][k] * fb[k];
+        }
+        mm[i] = a; me[i] = b; mf[i] = d;
+      }
+      for (int i = 0; i < 9; ++i) {
+        double s = (i >= 7) ? om : 1.0;
+        double d = s * (mm[i] - me[i]) - (1.0 - 0.5 * s) * mf[i];
+        g[i] = d * (1.0 / (double)ROWNORM[i]);
+      }
+      for (int k = 0; k < 9; ++k) {
+        double acc = 0.0;
+        for (int i = 0; i < 9; ++i) {
+          if (MM[i][k] == 0) continue;
+          acc = acc + (double)MM[i][k] * g[i];
+        }
+        po[k] = pn[k] - acc;
+      }
+    }
+  }
+}
+
+static void periodic_wrap(oracle_case* c) {
+  const int nx = c->nx_pad, ny = c->ny_pad;
+  if (c->x_periodic) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < ny; ++y)
+      for (int k = 0; k < 9; ++k) {
+        double l = c->pop[9 * ((long)(nx - 2) * ny + y) + k];
+        double r = c->pop[9 * ((long)1 * ny + y) + k];
+        c->pop[9 * ((long)0 * ny + y) + k] = l;
+        c->pop[9 * ((long)(nx - 1) * ny + y) + k] = r;
+      }
+  }
+  if (c->y_periodic) {
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; ++x)
+      for (int k = 0; k < 9; ++k) {
+        double b = c->pop[9 * ((long)x * ny + ny - 2) + k];
+        double t = c->pop[9 * ((long)x * ny + 1) + k];
+        c->pop[9 * ((long)x * ny + 0) + k] = b;
+        c->pop[9 * ((long)x * ny + ny - 1) + k] = t;
+      }
+  }
+}
+
+static void stream(oracle_case* c) {
+  const long size = (long)c->nx_pad * c->ny_pad;
+  const long ny = c->ny_pad;
+#pragma omp parallel for schedule(static)
+  for (long ind = 0; ind < size; ++ind) {
+    if (c->solid[ind] || c->ghost[ind]) continue;
+    long x = ind / ny, y = ind - x * ny;
+    double rho = c->density[ind];
+    c->pop_new[9 * ind] = c->pop[9 * ind];
+    for (int k = 1; k < 9; ++k) {
+      long nb = (x - CX[k]) * ny + (y - CY[k]);
+      if (!c->solid[nb]) {
+        c->pop_new[9 * ind + k] = c->pop[9 * nb + k];
+      } else {
+        int ki = INV[k];
+        double temp = ((2.0 * c->w[ki]) * rho) * c->inv_cs_2 *
+                      ((double)CX[ki] * c->velocity[2 * nb] +
+                       (double)CY[ki] * c->velocity[2 * nb + 1]);
+        c->pop_new[9 * ind + k] = c->pop[9 * ind + ki] - temp;
+      }
+    }
+  }
+}
+
+static void boundaries(oracle_case* c, const double* streamed) {
+  const long ny = c->ny_pad;
+  for (int e = 0; e < c->n_elements; ++e) {
+    const oracle_elem* el = &c->elems[e];
+    if (el->type == OR_PERIODIC) continue;
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < el->n_nodes; ++n) {
+      long ind = el->nodes[n];
+      if (c->solid[ind]) continue;
+      if (el->type == OR_BB) {
+        for (int q = 0; q < 3; ++q)
+          c->pop_new[9 * ind + el->inv[q]] = c->pop[9 * ind + el->out[q]];
+      } else if (el->type == OR_VEL) {
+        double rho = c->density[ind];
+        for (int q = 0; q < 3; ++q) {
+          int o = el->out[q];
+          double temp = ((2.0 * c->w[o]) * rho) * c->inv_cs_2 *
+                        ((double)CX[o] * el->vec[0] + (double)CY[o] * el->vec[1]);
+          c->pop_new[9 * ind + el->inv[q]] = c->pop[9 * ind + o] - temp;
+        }
+      } else if (el->type == OR_PRES) {
+        long nb = ind + (long)el->normal[0] * ny + el->normal[1];
+        double vx = c->velocity[2 * ind], vy = c->velocity[2 * ind + 1];
+        double ex = vx + 0.5 * (vx - c->velocity[2 * nb]);
+        double ey = vy + 0.5 * (vy - c->velocity[2 * nb + 1]);
+        double u2 = ex * ex + ey * ey;
+        for (int q = 0; q < 3; ++q) {
+          int o = el->out[q];
+          double cu = (double)CX[o] * ex + (double)CY[o] * ey;
+          double temp = ((2.0 * c->w[o]) * el->scalar) *
+                        (((1.0 + ((0.5 * c->inv_cs_4) * cu) * cu)) -
+                         (0.5 * c->inv_cs_2) * u2);
+          c->pop_new[9 * ind + el->inv[q]] = -c->pop[9 * ind + o] + temp;
+        }
+      } else if (el->type == OR_ZG) {
+        long nb = ind + (long)el->normal[0] * ny + el->normal[1];
+        for (int q = 0; q < 3; ++q) {
+          int v = el->inv[q];
+          c->pop_new[9 * ind + v] = streamed[9 * nb + v];
+        }
+      }
+    }
+  }
+}
+
+// Runs `steps` reference time steps with `n_threads` OpenMP threads (0 keeps
+// the runtime's default).  Returns 0.
+int oracle_run(oracle_case* c, int steps, int n_threads) {
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+  const long size = (long)c->nx_pad * c->ny_pad;
+  bool need_snapshot = false;
+  for (int e = 0; e < c->n_elements; ++e)
+    if (c->elems[e].type == OR_ZG) need_snapshot = true;
+  std::vector<double> snap(need_snapshot ? (size_t)size * 9 : 0);
+  for (int s = 0; s < steps; ++s) {
+    moments_and_collide(c);
+    periodic_wrap(c);
+    stream(c);
+    if (need_snapshot) std::memcpy(snap.data(), c->pop_new, sizeof(double) * size * 9);
+    boundaries(c, need_snapshot ? snap.data() : nullptr);
+  }
+  return 0;
+}
+
+int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+"""ctypes binding of libd2q9.so (include/d2q9.h).
+
+The library is loaded after ``import torch`` so that it binds to the HIP
+runtime torch already loaded (same soname): device pointers from torch
+tensors and torch streams are then valid in the library.  There is no CPU
+fallback: if the library is missing, or no GPU is visible, ``lib()`` raises.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede loading the HIP library)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libd2q9.so")
+
+STORE_F64, STORE_F32, STORE_F16DEV, STORE_BF16DEV = 0, 1, 2, 3
+BGK, MRT = 0, 1
+FORCING_NONE, GUO_LINEAR, GUO_SECOND_ORDER = 0, 1, 2
+FSRC_GRAVITY, FSRC_FIELD = 0, 1
+CELL_FLUID, CELL_SOLID, CELL_NEAR_SOLID, CELL_BC, CELL_STORE_MOM = (
+    1, 2, 4, 8, 16)
+RULE_STREAM, RULE_BB, RULE_VEL, RULE_PRES, RULE_ZG = 0, 1, 2, 3, 4
+
+STORAGE_CODES = {"f64": STORE_F64, "f32": STORE_F32,
+                 "f16dev": STORE_F16DEV, "bf16dev": STORE_BF16DEV}
+FORCING_CODES = {None: FORCING_NONE, "guo_linear": GUO_LINEAR,
+                 "guo_second_order": GUO_SECOND_ORDER}
+MODEL_CODES = {"BGK": BGK, "MRT": MRT}
+
+
+class BcElem(ctypes.Structure):
+    _fields_ = [("vec", ctypes.c_double * 2), ("scalar", ctypes.c_double),
+                ("normal", ctypes.c_int32 * 2), ("type", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+class BcNode(ctypes.Structure):
+    _fields_ = [("rule", ctypes.c_uint16 * 9), ("pad", ctypes.c_uint16)]
+
+
+class Lattice(ctypes.Structure):
+    """Mirror of ``d2q9_lattice``."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("pitch", ctypes.c_int32), ("origin", ctypes.c_int32),
+        ("plane", ctypes.c_int64),
+        ("storage", ctypes.c_int32), ("collision", ctypes.c_int32),
+        ("forcing", ctypes.c_int32), ("force_source", ctypes.c_int32),
+        ("x_periodic", ctypes.c_int32), ("y_periodic", ctypes.c_int32),
+        ("tile_j", ctypes.c_int32), ("tile_i", ctypes.c_int32),
+        ("omega", ctypes.c_double), ("inv_cs_2", ctypes.c_double),
+        ("inv_cs_4", ctypes.c_double), ("float_min", ctypes.c_double),
+        ("w", ctypes.c_double * 9), ("gravity", ctypes.c_double * 2),
+        ("rates", ctypes.c_double * 9), ("dev_scale", ctypes.c_double * 9),
+        ("flags", ctypes.c_void_p), ("tile_class", ctypes.c_void_p),
+        ("bc_slot", ctypes.c_void_p), ("bc_nodes", ctypes.c_void_p),
+        ("bc_elems", ctypes.c_void_p), ("force_field", ctypes.c_void_p),
+    ]
+
+
+assert ctypes.sizeof(BcElem) == 40 and ctypes.sizeof(BcNode) == 20
+
+_P = ctypes.c_void_p
+_LP = ctypes.POINTER(Lattice)
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "d2q9_init_eq": (ctypes.c_int, [_LP, _P, _P, _P, _P]),
+    "d2q9_collide": (ctypes.c_int, [_LP, _P, _P, _P, _P, ctypes.c_int, _P]),
+    "d2q9_stream_bc": (ctypes.c_int, [_LP, _P, _P, _P, _P]),
+    "d2q9_fused_step": (ctypes.c_int, [_LP, _P, _P, _P, _P, ctypes.c_int, _P]),
+    "d2q9_moments": (ctypes.c_int, [_LP, _P, _P, _P]),
+    "d2q9_reduce_work_len": (ctypes.c_int64, [_LP]),
+    "d2q9_mass_fp64": (ctypes.c_int, [_LP, _P, _P, _P, _P]),
+    "d2q9_residues": (ctypes.c_int, [_LP, _P, _P, _P, _P, _P]),
+    "d2q9_copy": (ctypes.c_int, [_P, _P, ctypes.c_int64, _P]),
+    "d2q9_graph_begin": (ctypes.c_int, [_P]),
+    "d2q9_graph_end": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_void_p)]),
+    "d2q9_graph_launch": (ctypes.c_int, [_P, _P]),
+    "d2q9_graph_destroy": (ctypes.c_int, [_P]),
+    "d2q9_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """Load the library and declare every prototype (no GPU needed)."""
+    if not os.path.exists(path):
+        raise NativeError(
+            f"{path} is missing: build it with `python -m pylabolt_amd.build`"
+        )
+    handle = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    return handle
+
+
+def lib():
+    """The loaded library; raises if it is missing or no GPU is visible."""
+    global _lib
+    if _lib is None:
+        if not torch.cuda.is_available():
+            raise NativeError("no HIP device visible: the D2Q9 engine runs on "
+                              "the GPU only")
+        _lib = load_library()
+    return _lib
+
+
+def check(rc, what):
+    """Raise ``NativeError`` with the library's message when rc != 0."""
+    if rc != 0:
+        msg = lib().d2q9_last_error().decode(errors="replace")
+        raise NativeError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(t):
+    """Device address of a tensor (or None)."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())

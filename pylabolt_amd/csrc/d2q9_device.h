@@ -1,0 +1,326 @@
+// Device building blocks of the D2Q9 engine for gfx950 (CDNA4).
+//
+// Shared by the single-lattice sweep (d2q9_step.hip), the batched persistent
+// sweep (d2q9_batched.hip) and the adjoint (d2q9_adjoint.hip).
+//
+// Arithmetic follows the reference kernels operation by operation, in the
+// association Python gives them (left to right), so that the FP64 path
+// reproduces the reference bit for bit when compiled with
+// -ffp-contract=off:
+//   equilibrium  pylabolt/parallel/cpu/equilibrium_kernels.py:28-35
+//   moments      pylabolt/parallel/cpu/compute_fields_kernels.py:22-65
+//   BGK + Guo    pylabolt/parallel/cpu/collision_kernels.py:37-153
+//   pull stream  pylabolt/parallel/cpu/streaming_kernels.py:29-47
+//   boundaries   pylabolt/parallel/cpu/fluid_boundary_kernels.py:5-152
+// The MRT operator, zero-gradient rule and per-cell force source have no
+// reference kernel; their operation order is the one oracle/d2q9_oracle.py
+// defines.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#include "../../include/d2q9.h"
+
+namespace d2q9 {
+
+// Velocity set, reference ordering (base/lattice.py:50-59).
+__device__ constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+__device__ constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+__device__ constexpr int INV[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+
+// Reference MRT moment matrix (base/collision_operator.py:147-157) and the
+// squared row norms that give M^-1 = M^T diag(1/norm).
+__device__ constexpr int MM[9][9] = {
+    {1, 1, 1, 1, 1, 1, 1, 1, 1},   {-4, -1, -1, -1, -1, 2, 2, 2, 2},
+    {4, -2, -2, -2, -2, 1, 1, 1, 1}, {0, 1, 0, -1, 0, 1, -1, -1, 1},
+    {0, -2, 0, 2, 0, 1, -1, -1, 1}, {0, 0, 1, 0, -1, 1, 1, -1, -1},
+    {0, 0, -2, 0, 2, 1, 1, -1, -1}, {0, 1, -1, 1, -1, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 1, -1, 1, -1}};
+__device__ constexpr int ROWNORM[9] = {9, 36, 36, 6, 12, 6, 12, 4, 4};
+
+// ---------------------------------------------------------------------------
+// Storage codecs: raw HBM element <-> compute value.
+// ---------------------------------------------------------------------------
+template <int S> struct Codec;
+
+template <> struct Codec<D2Q9_STORE_F64> {
+  using raw = double;
+  using T = double;
+  static __device__ __forceinline__ T dec(raw r, int, const d2q9_lattice&) { return r; }
+  static __device__ __forceinline__ raw enc(T v, int, const d2q9_lattice&) { return v; }
+  static __device__ __forceinline__ raw zero(int, const d2q9_lattice&) { return 0.0; }
+  static __device__ __forceinline__ double dec64(raw r, int, const d2q9_lattice&) { return r; }
+};
+
+template <> struct Codec<D2Q9_STORE_F32> {
+  using raw = float;
+  using T = float;
+  static __device__ __forceinline__ T dec(raw r, int, const d2q9_lattice&) { return r; }
+  static __device__ __forceinline__ raw enc(T v, int, const d2q9_lattice&) { return v; }
+  static __device__ __forceinline__ raw zero(int, const d2q9_lattice&) { return 0.0f; }
+  static __device__ __forceinline__ double dec64(raw r, int, const d2q9_lattice&) { return (double)r; }
+};
+
+// 16-bit forms store d = (f_k - w_k) * s_k; f_k = w_k + d / s_k.
+template <> struct Codec<D2Q9_STORE_F16DEV> {
+  using raw = __half;
+  using T = float;
+  static __device__ __forceinline__ T dec(raw r, int k, const d2q9_lattice& L) {
+    return (float)L.w[k] + __half2float(r) * (float)(1.0 / L.dev_scale[k]);
+  }
+  static __device__ __forceinline__ raw enc(T v, int k, const d2q9_lattice& L) {
+    return __float2half_rn((v - (float)L.w[k]) * (float)L.dev_scale[k]);
+  }
+  static __device__ __forceinline__ raw zero(int k, const d2q9_lattice& L) {
+    return __float2half_rn((0.0f - (float)L.w[k]) * (float)L.dev_scale[k]);
+  }
+  static __device__ __forceinline__ double dec64(raw r, int k, const d2q9_lattice& L) {
+    return L.w[k] + (double)__half2float(r) / L.dev_scale[k];
+  }
+};
+
+template <> struct Codec<D2Q9_STORE_BF16DEV> {
+  using raw = __hip_bfloat16;
+  using T = float;
+  static __device__ __forceinline__ T dec(raw r, int k, const d2q9_lattice& L) {
+    return (float)L.w[k] + __bfloat162float(r) * (float)(1.0 / L.dev_scale[k]);
+  }
+  static __device__ __forceinline__ raw enc(T v, int k, const d2q9_lattice& L) {
+    return __float2bfloat16((v - (float)L.w[k]) * (float)L.dev_scale[k]);
+  }
+  static __device__ __forceinline__ raw zero(int k, const d2q9_lattice& L) {
+    return __float2bfloat16((0.0f - (float)L.w[k]) * (float)L.dev_scale[k]);
+  }
+  static __device__ __forceinline__ double dec64(raw r, int k, const d2q9_lattice& L) {
+    return L.w[k] + (double)__bfloat162float(r) / L.dev_scale[k];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Equilibrium and Guo source, reference association.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T feq_k(int k, T rho, T ux, T uy, T u2, T wk,
+                                   T ics2, T ics4) {
+  T cu = (T)CX[k] * ux + (T)CY[k] * uy;
+  return (wk * rho) *
+         (((T)1 + ics2 * cu + ((T)0.5 * ics4) * cu * cu) - ((T)0.5 * ics2) * u2);
+}
+
+// Guo source without the (1 - omega/2) factor.
+template <typename T, int FORCING>
+__device__ __forceinline__ T guo_k(int k, T ux, T uy, T fx, T fy, T wk,
+                                   T ics2, T ics4) {
+  if constexpr (FORCING == D2Q9_GUO_LINEAR) {
+    return (wk * ((T)CX[k] * fx + (T)CY[k] * fy)) * ics2;
+  } else {
+    T cu = (T)CX[k] * ux + (T)CY[k] * uy;
+    T const_x = ((T)CX[k] - ux) * ics2 + (cu * (T)CX[k]) * ics4;
+    T const_y = ((T)CY[k] - uy) * ics2 + (cu * (T)CY[k]) * ics4;
+    return wk * (const_x * fx + const_y * fy);
+  }
+}
+
+// Moments of the pre-collision populations g (fluid cell):
+// rho = sum g (from 0), F = rho * gravity or the per-cell field,
+// u = (sum c g) * inv + (0.5 F) * inv with inv = 1 / (rho + float_min).
+template <typename T, int FORCING>
+__device__ __forceinline__ void moments(const T (&g)[9], const d2q9_lattice& L,
+                                        T ffx, T ffy, T& rho, T& ux, T& uy,
+                                        T& fx, T& fy) {
+  T r = (T)0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) r += g[k];
+  rho = r;
+  const bool field = L.force_source == D2Q9_FSRC_FIELD;
+  if (field) {
+    fx = ffx;
+    fy = ffy;
+  } else {
+    fx = rho * (T)L.gravity[0];
+    fy = rho * (T)L.gravity[1];
+  }
+  T sx = (T)0, sy = (T)0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    sx += (T)CX[k] * g[k];
+    sy += (T)CY[k] * g[k];
+  }
+  T inv = (T)1 / (rho + (T)L.float_min);
+  if (FORCING != D2Q9_FORCING_NONE || field) {
+    ux = sx * inv + ((T)0.5 * fx) * inv;
+    uy = sy * inv + ((T)0.5 * fy) * inv;
+  } else {
+    // gravity is zero when forcing is None (base/force_operator.py:58-69),
+    // so the force term adds exactly zero.
+    ux = sx * inv;
+    uy = sy * inv;
+  }
+}
+
+// BGK collision in place: g <- (1 - om) g + om feq (+ (1 - om/2) Fbar).
+template <typename T, int FORCING>
+__device__ __forceinline__ void collide_bgk(T (&g)[9], const d2q9_lattice& L,
+                                            T rho, T ux, T uy, T fx, T fy) {
+  const T om = (T)L.omega;
+  const T ics2 = (T)L.inv_cs_2, ics4 = (T)L.inv_cs_4;
+  const T u2 = ux * ux + uy * uy;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    T wk = (T)L.w[k];
+    T feq = feq_k<T>(k, rho, ux, uy, u2, wk, ics2, ics4);
+    T v = ((T)1 - om) * g[k] + om * feq;
+    if constexpr (FORCING != D2Q9_FORCING_NONE) {
+      v = v + ((T)1 - (T)0.5 * om) * guo_k<T, FORCING>(k, ux, uy, fx, fy, wk, ics2, ics4);
+    }
+    g[k] = v;
+  }
+}
+
+// MRT with Guo forcing in place (operation order of oracle mrt_collide).
+template <typename T, int FORCING>
+__device__ __forceinline__ void collide_mrt(T (&g)[9], const d2q9_lattice& L,
+                                            T rho, T ux, T uy, T fx, T fy) {
+  const T ics2 = (T)L.inv_cs_2, ics4 = (T)L.inv_cs_4;
+  const T u2 = ux * ux + uy * uy;
+  T feq[9], fb[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    T wk = (T)L.w[k];
+    feq[k] = feq_k<T>(k, rho, ux, uy, u2, wk, ics2, ics4);
+    if constexpr (FORCING != D2Q9_FORCING_NONE)
+      fb[k] = guo_k<T, FORCING>(k, ux, uy, fx, fy, wk, ics2, ics4);
+    else
+      fb[k] = (T)0;
+  }
+  T gm[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    T a = (T)0, b = (T)0, d = (T)0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      if (MM[i][k] == 0) continue;
+      a = a + (T)MM[i][k] * g[k];
+      b = b + (T)MM[i][k] * feq[k];
+      if constexpr (FORCING != D2Q9_FORCING_NONE) d = d + (T)MM[i][k] * fb[k];
+    }
+    T s = (T)L.rates[i];
+    T dm = s * (a - b) - ((T)1 - (T)0.5 * s) * d;
+    gm[i] = dm * (T)(1.0 / (double)ROWNORM[i]);
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    T acc = (T)0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      if (MM[i][k] == 0) continue;
+      acc = acc + (T)MM[i][k] * gm[i];
+    }
+    g[k] = g[k] - acc;
+  }
+}
+
+template <typename T, int MODEL, int FORCING>
+__device__ __forceinline__ void collide(T (&g)[9], const d2q9_lattice& L, T rho,
+                                        T ux, T uy, T fx, T fy) {
+  if constexpr (MODEL == D2Q9_MRT)
+    collide_mrt<T, FORCING>(g, L, rho, ux, uy, fx, fy);
+  else
+    collide_bgk<T, FORCING>(g, L, rho, ux, uy, fx, fy);
+}
+
+// ---------------------------------------------------------------------------
+// Streaming and boundary rules on the general (flagged) path.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t cell_index(const d2q9_lattice& L, int i, int j) {
+  return (int64_t)L.origin + (int64_t)i * L.pitch + j;
+}
+
+// Value streamed into cell (i, j) in direction k (streaming_kernels.py:35-47
+// with the periodic wrap of MPI_kernels.py:34-99 folded into the index):
+// from the pull neighbour, or moving-wall bounce-back off a solid neighbour.
+template <int S>
+__device__ __forceinline__ typename Codec<S>::T pull(
+    const d2q9_lattice& L, const typename Codec<S>::raw* __restrict__ f,
+    const typename Codec<S>::T* __restrict__ mom, int i, int j, int64_t c,
+    uint8_t fl, int k) {
+  using T = typename Codec<S>::T;
+  int xn = i - CX[k], yn = j - CY[k];
+  if (fl & D2Q9_CELL_NEAR_SOLID) {
+    int64_t nb = cell_index(L, xn, yn);
+    if (L.flags[nb] & D2Q9_CELL_SOLID) {
+      const int ki = INV[k];
+      T rho_c = mom[c];
+      T usx = mom[L.plane + nb], usy = mom[2 * L.plane + nb];
+      T temp = ((((T)2 * (T)L.w[ki]) * rho_c) * (T)L.inv_cs_2) *
+               ((T)CX[ki] * usx + (T)CY[ki] * usy);
+      return Codec<S>::dec(f[(int64_t)ki * L.plane + c], ki, L) - temp;
+    }
+  }
+  if (L.x_periodic) {
+    if (xn == 0) xn = L.nx;
+    else if (xn == L.nx + 1) xn = 1;
+  }
+  if (L.y_periodic) {
+    if (yn == 0) yn = L.ny;
+    else if (yn == L.ny + 1) yn = 1;
+  }
+  return Codec<S>::dec(f[(int64_t)k * L.plane + cell_index(L, xn, yn)], k, L);
+}
+
+// Overwrite the rule-bearing directions of boundary node (i, j)
+// (fluid_boundary_kernels.py:5-152; rules resolved in element order on the
+// host, so the last element writing a direction is the one stored).
+template <int S>
+__device__ __forceinline__ void apply_bc(
+    const d2q9_lattice& L, const typename Codec<S>::raw* __restrict__ f,
+    const typename Codec<S>::T* __restrict__ mom, int i, int j, int64_t c,
+    typename Codec<S>::T (&g)[9]) {
+  using T = typename Codec<S>::T;
+  const d2q9_bc_node node = L.bc_nodes[L.bc_slot[c]];
+  const T ics2 = (T)L.inv_cs_2, ics4 = (T)L.inv_cs_4;
+#pragma unroll
+  for (int d = 1; d < 9; ++d) {
+    const int rule = node.rule[d];
+    const int kind = rule & 15;
+    if (kind == D2Q9_RULE_STREAM) continue;
+    const d2q9_bc_elem& e = L.bc_elems[rule >> 4];
+    const int o = INV[d];
+    if (kind == D2Q9_RULE_BB) {
+      g[d] = Codec<S>::dec(f[(int64_t)o * L.plane + c], o, L);
+    } else if (kind == D2Q9_RULE_VEL) {
+      T rho_c = mom[c];
+      T temp = ((((T)2 * (T)L.w[o]) * rho_c) * ics2) *
+               ((T)CX[o] * (T)e.vec[0] + (T)CY[o] * (T)e.vec[1]);
+      g[d] = Codec<S>::dec(f[(int64_t)o * L.plane + c], o, L) - temp;
+    } else if (kind == D2Q9_RULE_PRES) {
+      int64_t nb = c + (int64_t)e.normal[0] * L.pitch + e.normal[1];
+      T vx = mom[L.plane + c], vy = mom[2 * L.plane + c];
+      T ex = vx + (T)0.5 * (vx - mom[L.plane + nb]);
+      T ey = vy + (T)0.5 * (vy - mom[2 * L.plane + nb]);
+      T u2 = ex * ex + ey * ey;
+      T cu = (T)CX[o] * ex + (T)CY[o] * ey;
+      T temp = (((T)2 * (T)L.w[o]) * (T)e.scalar) *
+               (((T)1 + ((T)0.5 * ics4) * cu * cu) - ((T)0.5 * ics2) * u2);
+      g[d] = -Codec<S>::dec(f[(int64_t)o * L.plane + c], o, L) + temp;
+    } else {  // D2Q9_RULE_ZG
+      int ib = i + e.normal[0], jb = j + e.normal[1];
+      int64_t nb = cell_index(L, ib, jb);
+      uint8_t fnb = L.flags[nb];
+      g[d] = (fnb & D2Q9_CELL_FLUID) ? pull<S>(L, f, mom, ib, jb, nb, fnb, d) : (T)0;
+    }
+  }
+}
+
+// Workgroup b is dispatched to XCD b % 8; give each XCD a contiguous run of
+// tiles so that tiles sharing cache lines meet in the same L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  const int per = n >> 3, rem = n & 7;
+  const int x = b & 7, r = b >> 3;
+  return x * per + (x < rem ? x : rem) + r;
+}
+
+}  // namespace d2q9

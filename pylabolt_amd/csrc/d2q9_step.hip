@@ -1,0 +1,506 @@
+// Single-lattice D2Q9 sweep kernels and their C ABI (include/d2q9.h).
+//
+// One thread per interior cell; a 256-thread workgroup covers a tile of
+// tile_j cells along y (contiguous in memory) by tile_i rows along x.  Every
+// population plane is read at a shifted position (pull streaming) and
+// written unshifted, so a wave's 64 lanes read and write 64 consecutive
+// elements of each of the 9 planes: every HBM access is coalesced.
+//
+// Tiles whose cells are all plain fluid (no solid neighbour, no boundary
+// rule, no periodic edge, no moment the next step reads) take a branch-free
+// path that never touches the flag array; the others read the per-cell flag
+// byte and apply bounce-back, periodic wrap and boundary rules in the same
+// sweep.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "d2q9_device.h"
+
+using namespace d2q9;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* what) {
+  g_last_error = what;
+  return code;
+}
+
+int check_hip(hipError_t e, const char* where) {
+  if (e == hipSuccess) return 0;
+  g_last_error = std::string(where) + ": " + hipGetErrorString(e);
+  return (int)e;
+}
+
+enum { PHASE_COLLIDE = 0, PHASE_FUSED = 1, PHASE_STREAM = 2 };
+
+template <int S, int MODEL, int FORCING, int PHASE>
+__global__ __launch_bounds__(256) void k_sweep(
+    const d2q9_lattice L, const typename Codec<S>::raw* __restrict__ fsrc,
+    typename Codec<S>::raw* __restrict__ fdst,
+    const typename Codec<S>::T* __restrict__ msrc,
+    typename Codec<S>::T* __restrict__ mdst, int write_mom, int ntj,
+    int ntiles) {
+  using T = typename Codec<S>::T;
+  const int t = xcd_remap(blockIdx.x, ntiles);
+  const int tj = t % ntj, ti = t / ntj;
+  const int j = 1 + tj * L.tile_j + (int)threadIdx.x;
+  const int i = 1 + ti * L.tile_i + (int)threadIdx.y;
+  if (i > L.nx || j > L.ny) return;
+  const int64_t P = L.plane;
+  const int64_t c = cell_index(L, i, j);
+  const bool plain = L.tile_class != nullptr && L.tile_class[t] == 0;
+
+  T g[9];
+  uint8_t fl = D2Q9_CELL_FLUID;
+  if (plain) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      int64_t src = c;
+      if (PHASE != PHASE_COLLIDE) src = c - (int64_t)CX[k] * L.pitch - CY[k];
+      g[k] = Codec<S>::dec(fsrc[(int64_t)k * P + src], k, L);
+    }
+  } else {
+    fl = L.flags[c];
+    if (!(fl & D2Q9_CELL_FLUID)) return;
+    if (PHASE == PHASE_COLLIDE) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) g[k] = Codec<S>::dec(fsrc[(int64_t)k * P + c], k, L);
+    } else {
+      g[0] = Codec<S>::dec(fsrc[c], 0, L);
+#pragma unroll
+      for (int k = 1; k < 9; ++k) g[k] = pull<S>(L, fsrc, msrc, i, j, c, fl, k);
+      if (fl & D2Q9_CELL_BC) apply_bc<S>(L, fsrc, msrc, i, j, c, g);
+    }
+  }
+
+  if constexpr (PHASE == PHASE_STREAM) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) fdst[(int64_t)k * P + c] = Codec<S>::enc(g[k], k, L);
+    return;
+  } else {
+    T ffx = (T)0, ffy = (T)0;
+    if (L.force_source == D2Q9_FSRC_FIELD) {
+      const T* ff = static_cast<const T*>(L.force_field);
+      ffx = ff[c];
+      ffy = ff[P + c];
+    }
+    T rho, ux, uy, fx, fy;
+    moments<T, FORCING>(g, L, ffx, ffy, rho, ux, uy, fx, fy);
+    collide<T, MODEL, FORCING>(g, L, rho, ux, uy, fx, fy);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) fdst[(int64_t)k * P + c] = Codec<S>::enc(g[k], k, L);
+    if (write_mom || (fl & D2Q9_CELL_STORE_MOM)) {
+      mdst[c] = rho;
+      mdst[P + c] = ux;
+      mdst[2 * P + c] = uy;
+    }
+  }
+}
+
+// f = feq(rho0, u0) on fluid cells, encoded zero on every other padded cell.
+template <int S>
+__global__ __launch_bounds__(256) void k_init(const d2q9_lattice L,
+                                              typename Codec<S>::raw* fa,
+                                              typename Codec<S>::raw* fb,
+                                              const typename Codec<S>::T* mom) {
+  using T = typename Codec<S>::T;
+  const int64_t n = (int64_t)(L.nx + 2) * L.pitch;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int i = (int)(e / L.pitch), j = (int)(e % L.pitch);
+  if (j > L.ny + 1) return;
+  const int64_t c = cell_index(L, i, j);
+  const int64_t P = L.plane;
+  if (L.flags[c] & D2Q9_CELL_FLUID) {
+    const T rho = mom[c], ux = mom[P + c], uy = mom[2 * P + c];
+    const T ics2 = (T)L.inv_cs_2, ics4 = (T)L.inv_cs_4;
+    const T u2 = ux * ux + uy * uy;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      auto r = Codec<S>::enc(feq_k<T>(k, rho, ux, uy, u2, (T)L.w[k], ics2, ics4), k, L);
+      fa[k * P + c] = r;
+      fb[k * P + c] = r;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      auto z = Codec<S>::zero(k, L);
+      fa[k * P + c] = z;
+      fb[k * P + c] = z;
+    }
+  }
+}
+
+// rho/u of the stored populations on every fluid cell (no force term: these
+// are populations after streaming; the force of the next step is added by
+// the sweep itself).
+template <int S>
+__global__ __launch_bounds__(256) void k_moments(const d2q9_lattice L,
+                                                 const typename Codec<S>::raw* f,
+                                                 typename Codec<S>::T* mom) {
+  using T = typename Codec<S>::T;
+  const int64_t n = (int64_t)L.nx * L.ny;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int i = 1 + (int)(e / L.ny), j = 1 + (int)(e % L.ny);
+  const int64_t c = cell_index(L, i, j);
+  if (!(L.flags[c] & D2Q9_CELL_FLUID)) return;
+  const int64_t P = L.plane;
+  T g[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) g[k] = Codec<S>::dec(f[k * P + c], k, L);
+  T ffx = (T)0, ffy = (T)0;
+  if (L.force_source == D2Q9_FSRC_FIELD) {
+    const T* ff = static_cast<const T*>(L.force_field);
+    ffx = ff[c];
+    ffy = ff[P + c];
+  }
+  T rho, ux, uy, fx, fy;
+  if (L.forcing == D2Q9_FORCING_NONE)
+    moments<T, D2Q9_FORCING_NONE>(g, L, ffx, ffy, rho, ux, uy, fx, fy);
+  else
+    moments<T, D2Q9_GUO_LINEAR>(g, L, ffx, ffy, rho, ux, uy, fx, fy);
+  mom[c] = rho;
+  mom[P + c] = ux;
+  mom[2 * P + c] = uy;
+}
+
+// ---- FP64 reductions (deterministic two-level tree) ----------------------
+constexpr int RED_THREADS = 256;
+constexpr int RED_MAX_BLOCKS = 2048;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  // wave-level tree then across the 4 waves of the workgroup
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < RED_THREADS / 64; ++w) r += sh[w];
+  }
+  __syncthreads();
+  return r;
+}
+
+template <int S>
+__global__ __launch_bounds__(RED_THREADS) void k_mass_partial(
+    const d2q9_lattice L, const typename Codec<S>::raw* f, double* partial) {
+  __shared__ double sh[RED_THREADS / 64];
+  const int64_t n = (int64_t)L.nx * L.ny;
+  double acc = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * RED_THREADS) {
+    const int i = 1 + (int)(e / L.ny), j = 1 + (int)(e % L.ny);
+    const int64_t c = cell_index(L, i, j);
+    if (!(L.flags[c] & D2Q9_CELL_FLUID)) continue;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s += Codec<S>::dec64(f[k * L.plane + c], k, L);
+    acc += s;
+  }
+  double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+template <int NV>
+__global__ __launch_bounds__(RED_THREADS) void k_finish(const double* partial,
+                                                        int nblocks, double* out) {
+  __shared__ double sh[RED_THREADS / 64];
+  for (int q = 0; q < NV; ++q) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += RED_THREADS) acc += partial[q * RED_MAX_BLOCKS + b];
+    double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[q] = r;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_residue_partial(
+    const d2q9_lattice L, const T* mom, T* mom_old, double* partial) {
+  __shared__ double sh[RED_THREADS / 64];
+  const int64_t n = (int64_t)L.nx * L.ny;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t e = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * RED_THREADS) {
+    const int i = 1 + (int)(e / L.ny), j = 1 + (int)(e % L.ny);
+    const int64_t c = cell_index(L, i, j);
+    if (!(L.flags[c] & D2Q9_CELL_FLUID)) continue;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double v = (double)mom[q * L.plane + c];
+      double o = (double)mom_old[q * L.plane + c];
+      double d = v - o;
+      acc[2 * q] += d * d;
+      acc[2 * q + 1] += o * o;
+      mom_old[q * L.plane + c] = mom[q * L.plane + c];
+    }
+  }
+  for (int q = 0; q < 6; ++q) {
+    double r = block_sum(acc[q], sh);
+    if (threadIdx.x == 0) partial[q * RED_MAX_BLOCKS + blockIdx.x] = r;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src,
+                                                uint4* __restrict__ dst, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    dst[e] = src[e];
+}
+
+int validate(const d2q9_lattice* L) {
+  if (!L) return fail(D2Q9_E_ARG, "null lattice descriptor");
+  if (L->nx < 1 || L->ny < 1) return fail(D2Q9_E_ARG, "nx and ny must be >= 1");
+  if (L->pitch < L->ny + 2) return fail(D2Q9_E_ARG, "pitch < ny + 2");
+  if (L->origin < 0 || L->plane < (int64_t)L->origin + (int64_t)(L->nx + 2) * L->pitch)
+    return fail(D2Q9_E_ARG, "plane too small for origin + (nx + 2) * pitch");
+  if (L->tile_j < 1 || L->tile_i < 1 || L->tile_j * L->tile_i != 256)
+    return fail(D2Q9_E_ARG, "tile_j * tile_i must be 256");
+  if (L->storage < 0 || L->storage > 3) return fail(D2Q9_E_ARG, "bad storage");
+  if (L->collision < 0 || L->collision > 1) return fail(D2Q9_E_ARG, "bad collision");
+  if (L->forcing < 0 || L->forcing > 2) return fail(D2Q9_E_ARG, "bad forcing");
+  if (L->force_source == D2Q9_FSRC_FIELD && !L->force_field)
+    return fail(D2Q9_E_ARG, "force_source FIELD needs force_field");
+  if (!L->flags) return fail(D2Q9_E_ARG, "null flags");
+  if ((L->bc_nodes == nullptr) != (L->bc_slot == nullptr))
+    return fail(D2Q9_E_ARG, "bc_nodes and bc_slot go together");
+  return 0;
+}
+
+template <int S, int MODEL, int FORCING, int PHASE>
+int launch_sweep(const d2q9_lattice* L, const void* fsrc, void* fdst,
+                 const void* msrc, void* mdst, int write_mom, hipStream_t s) {
+  using C = Codec<S>;
+  const int ntj = (L->ny + L->tile_j - 1) / L->tile_j;
+  const int nti = (L->nx + L->tile_i - 1) / L->tile_i;
+  const int64_t ntiles = (int64_t)ntj * nti;
+  if (ntiles > 0x7fffffff) return fail(D2Q9_E_ARG, "lattice too large");
+  dim3 block(L->tile_j, L->tile_i);
+  hipLaunchKernelGGL((k_sweep<S, MODEL, FORCING, PHASE>), dim3((unsigned)ntiles),
+                     block, 0, s, *L, static_cast<const typename C::raw*>(fsrc),
+                     static_cast<typename C::raw*>(fdst),
+                     static_cast<const typename C::T*>(msrc),
+                     static_cast<typename C::T*>(mdst), write_mom, ntj, (int)ntiles);
+  return check_hip(hipGetLastError(), "k_sweep launch");
+}
+
+template <int S, int PHASE>
+int dispatch_model(const d2q9_lattice* L, const void* a, void* b, const void* c,
+                   void* d, int wm, hipStream_t s) {
+  const int key = L->collision * 3 + L->forcing;
+  switch (key) {
+    case 0: return launch_sweep<S, D2Q9_BGK, D2Q9_FORCING_NONE, PHASE>(L, a, b, c, d, wm, s);
+    case 1: return launch_sweep<S, D2Q9_BGK, D2Q9_GUO_LINEAR, PHASE>(L, a, b, c, d, wm, s);
+    case 2: return launch_sweep<S, D2Q9_BGK, D2Q9_GUO_SECOND_ORDER, PHASE>(L, a, b, c, d, wm, s);
+    case 3: return launch_sweep<S, D2Q9_MRT, D2Q9_FORCING_NONE, PHASE>(L, a, b, c, d, wm, s);
+    case 4: return launch_sweep<S, D2Q9_MRT, D2Q9_GUO_LINEAR, PHASE>(L, a, b, c, d, wm, s);
+    case 5: return launch_sweep<S, D2Q9_MRT, D2Q9_GUO_SECOND_ORDER, PHASE>(L, a, b, c, d, wm, s);
+  }
+  return fail(D2Q9_E_ARG, "bad collision/forcing");
+}
+
+template <int PHASE>
+int dispatch(const d2q9_lattice* L, const void* fsrc, void* fdst, const void* msrc,
+             void* mdst, int wm, void* stream) {
+  if (int r = validate(L)) return r;
+  if (!fsrc || !fdst || !msrc) return fail(D2Q9_E_ARG, "null buffer");
+  if (PHASE != PHASE_STREAM && !mdst) return fail(D2Q9_E_ARG, "null mom_dst");
+  if (fsrc == fdst) return fail(D2Q9_E_ARG, "f_src and f_dst must differ");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (PHASE == PHASE_STREAM) {
+    // streaming never touches the collision model: one instantiation each
+    switch (L->storage) {
+      case D2Q9_STORE_F64: return launch_sweep<D2Q9_STORE_F64, D2Q9_BGK, D2Q9_FORCING_NONE, PHASE>(L, fsrc, fdst, msrc, mdst, 0, s);
+      case D2Q9_STORE_F32: return launch_sweep<D2Q9_STORE_F32, D2Q9_BGK, D2Q9_FORCING_NONE, PHASE>(L, fsrc, fdst, msrc, mdst, 0, s);
+      case D2Q9_STORE_F16DEV: return launch_sweep<D2Q9_STORE_F16DEV, D2Q9_BGK, D2Q9_FORCING_NONE, PHASE>(L, fsrc, fdst, msrc, mdst, 0, s);
+      case D2Q9_STORE_BF16DEV: return launch_sweep<D2Q9_STORE_BF16DEV, D2Q9_BGK, D2Q9_FORCING_NONE, PHASE>(L, fsrc, fdst, msrc, mdst, 0, s);
+    }
+  } else {
+    switch (L->storage) {
+      case D2Q9_STORE_F64: return dispatch_model<D2Q9_STORE_F64, PHASE>(L, fsrc, fdst, msrc, mdst, wm, s);
+      case D2Q9_STORE_F32: return dispatch_model<D2Q9_STORE_F32, PHASE>(L, fsrc, fdst, msrc, mdst, wm, s);
+      case D2Q9_STORE_F16DEV: return dispatch_model<D2Q9_STORE_F16DEV, PHASE>(L, fsrc, fdst, msrc, mdst, wm, s);
+      case D2Q9_STORE_BF16DEV: return dispatch_model<D2Q9_STORE_BF16DEV, PHASE>(L, fsrc, fdst, msrc, mdst, wm, s);
+    }
+  }
+  return fail(D2Q9_E_ARG, "bad storage");
+}
+
+int red_blocks(const d2q9_lattice* L) {
+  int64_t n = (int64_t)L->nx * L->ny;
+  int64_t b = (n + RED_THREADS * 8 - 1) / (RED_THREADS * 8);
+  if (b < 1) b = 1;
+  if (b > RED_MAX_BLOCKS) b = RED_MAX_BLOCKS;
+  return (int)b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int d2q9_init_eq(const d2q9_lattice* L, void* f_a, void* f_b, const void* mom,
+                 void* stream) {
+  if (int r = validate(L)) return r;
+  if (!f_a || !f_b || !mom) return fail(D2Q9_E_ARG, "null buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t n = (int64_t)(L->nx + 2) * L->pitch;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  switch (L->storage) {
+#define D2Q9_INIT(SV)                                                          \
+  case SV:                                                                     \
+    hipLaunchKernelGGL((k_init<SV>), dim3(blocks), dim3(256), 0, s, *L,        \
+                       static_cast<Codec<SV>::raw*>(f_a),                      \
+                       static_cast<Codec<SV>::raw*>(f_b),                      \
+                       static_cast<const Codec<SV>::T*>(mom));                 \
+    break;
+    D2Q9_INIT(D2Q9_STORE_F64)
+    D2Q9_INIT(D2Q9_STORE_F32)
+    D2Q9_INIT(D2Q9_STORE_F16DEV)
+    D2Q9_INIT(D2Q9_STORE_BF16DEV)
+#undef D2Q9_INIT
+    default: return fail(D2Q9_E_ARG, "bad storage");
+  }
+  return check_hip(hipGetLastError(), "k_init launch");
+}
+
+int d2q9_collide(const d2q9_lattice* L, const void* f_src, void* f_dst,
+                 const void* mom_src, void* mom_dst, int write_moments,
+                 void* stream) {
+  return dispatch<PHASE_COLLIDE>(L, f_src, f_dst, mom_src, mom_dst, write_moments, stream);
+}
+
+int d2q9_stream_bc(const d2q9_lattice* L, const void* f_src, void* f_dst,
+                   const void* mom_src, void* stream) {
+  return dispatch<PHASE_STREAM>(L, f_src, f_dst, mom_src, nullptr, 0, stream);
+}
+
+int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
+                    const void* mom_src, void* mom_dst, int write_moments,
+                    void* stream) {
+  if (mom_src == mom_dst) return fail(D2Q9_E_ARG, "mom_src and mom_dst must differ");
+  return dispatch<PHASE_FUSED>(L, f_src, f_dst, mom_src, mom_dst, write_moments, stream);
+}
+
+int d2q9_moments(const d2q9_lattice* L, const void* f, void* mom, void* stream) {
+  if (int r = validate(L)) return r;
+  if (!f || !mom) return fail(D2Q9_E_ARG, "null buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t n = (int64_t)L->nx * L->ny;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  switch (L->storage) {
+#define D2Q9_MOM(SV)                                                           \
+  case SV:                                                                     \
+    hipLaunchKernelGGL((k_moments<SV>), dim3(blocks), dim3(256), 0, s, *L,     \
+                       static_cast<const Codec<SV>::raw*>(f),                  \
+                       static_cast<Codec<SV>::T*>(mom));                       \
+    break;
+    D2Q9_MOM(D2Q9_STORE_F64)
+    D2Q9_MOM(D2Q9_STORE_F32)
+    D2Q9_MOM(D2Q9_STORE_F16DEV)
+    D2Q9_MOM(D2Q9_STORE_BF16DEV)
+#undef D2Q9_MOM
+    default: return fail(D2Q9_E_ARG, "bad storage");
+  }
+  return check_hip(hipGetLastError(), "k_moments launch");
+}
+
+int64_t d2q9_reduce_work_len(const d2q9_lattice* L) {
+  (void)L;
+  return (int64_t)6 * RED_MAX_BLOCKS;
+}
+
+int d2q9_mass_fp64(const d2q9_lattice* L, const void* f, double* work,
+                   double* out, void* stream) {
+  if (int r = validate(L)) return r;
+  if (!f || !work || !out) return fail(D2Q9_E_ARG, "null buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = red_blocks(L);
+  switch (L->storage) {
+#define D2Q9_MASS(SV)                                                          \
+  case SV:                                                                     \
+    hipLaunchKernelGGL((k_mass_partial<SV>), dim3(nb), dim3(RED_THREADS), 0,  \
+                       s, *L, static_cast<const Codec<SV>::raw*>(f), work);    \
+    break;
+    D2Q9_MASS(D2Q9_STORE_F64)
+    D2Q9_MASS(D2Q9_STORE_F32)
+    D2Q9_MASS(D2Q9_STORE_F16DEV)
+    D2Q9_MASS(D2Q9_STORE_BF16DEV)
+#undef D2Q9_MASS
+    default: return fail(D2Q9_E_ARG, "bad storage");
+  }
+  hipLaunchKernelGGL((k_finish<1>), dim3(1), dim3(RED_THREADS), 0, s, work, nb, out);
+  return check_hip(hipGetLastError(), "k_mass launch");
+}
+
+int d2q9_residues(const d2q9_lattice* L, const void* mom, void* mom_old,
+                  double* work, double* out, void* stream) {
+  if (int r = validate(L)) return r;
+  if (!mom || !mom_old || !work || !out) return fail(D2Q9_E_ARG, "null buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = red_blocks(L);
+  if (L->storage == D2Q9_STORE_F64) {
+    hipLaunchKernelGGL((k_residue_partial<double>), dim3(nb), dim3(RED_THREADS), 0, s,
+                       *L, static_cast<const double*>(mom), static_cast<double*>(mom_old), work);
+  } else {
+    hipLaunchKernelGGL((k_residue_partial<float>), dim3(nb), dim3(RED_THREADS), 0, s,
+                       *L, static_cast<const float*>(mom), static_cast<float*>(mom_old), work);
+  }
+  hipLaunchKernelGGL((k_finish<6>), dim3(1), dim3(RED_THREADS), 0, s, work, nb, out);
+  return check_hip(hipGetLastError(), "k_residue launch");
+}
+
+int d2q9_copy(void* dst, const void* src, int64_t bytes, void* stream) {
+  if (!dst || !src || bytes < 0 || (bytes % 16) != 0)
+    return fail(D2Q9_E_ARG, "copy needs non-null buffers and a multiple of 16 bytes");
+  const int64_t n = bytes / 16;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const uint4*>(src),
+                     static_cast<uint4*>(dst), n);
+  return check_hip(hipGetLastError(), "k_copy16 launch");
+}
+
+int d2q9_graph_begin(void* stream) {
+  return check_hip(hipStreamBeginCapture(static_cast<hipStream_t>(stream),
+                                         hipStreamCaptureModeThreadLocal),
+                   "hipStreamBeginCapture");
+}
+
+int d2q9_graph_end(void* stream, void** graph_exec) {
+  if (!graph_exec) return fail(D2Q9_E_ARG, "null graph_exec");
+  hipGraph_t g = nullptr;
+  if (int r = check_hip(hipStreamEndCapture(static_cast<hipStream_t>(stream), &g),
+                        "hipStreamEndCapture"))
+    return r;
+  hipGraphExec_t ge = nullptr;
+  int r = check_hip(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+  (void)hipGraphDestroy(g);
+  if (r) return r;
+  *graph_exec = ge;
+  return 0;
+}
+
+int d2q9_graph_launch(void* graph_exec, void* stream) {
+  if (!graph_exec) return fail(D2Q9_E_ARG, "null graph_exec");
+  return check_hip(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec),
+                                  static_cast<hipStream_t>(stream)),
+                   "hipGraphLaunch");
+}
+
+int d2q9_graph_destroy(void* graph_exec) {
+  if (!graph_exec) return 0;
+  return check_hip(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)),
+                   "hipGraphExecDestroy");
+}
+
+const char* d2q9_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,482 @@
+"""The single-lattice D2Q9 engine: device buffers, cell tables, step control.
+
+HBM layout (one lattice):
+
+* populations: two buffers (ping-pong), each ``9 * plane`` elements of the
+  storage type, structure of arrays: direction ``k`` occupies
+  ``[k * plane, (k + 1) * plane)``;
+* inside a plane, padded cell ``(i, j)`` (``0 <= i <= nx+1``,
+  ``0 <= j <= ny+1``, ghost ring included) sits at
+  ``origin + i * pitch + j`` with ``origin = 63`` and ``pitch`` a multiple
+  of 64, so each interior row ``j = 1..ny`` starts on a 64-element (512 B
+  in FP64) boundary;
+* moments: two buffers of three planes ``[rho | ux | uy]`` in the compute
+  type (double for FP64 storage, float otherwise);
+* one flag byte per padded cell, a per-tile class byte, and the boundary
+  tables (a slot per boundary node, one rule per direction).
+
+Step bookkeeping.  The reference step n computes moments and collision from
+``pop_new[n-1]`` into ``pop[n]``, then streams and applies boundaries into
+``pop_new[n]`` (pylabolt/solvers/fluidLB.py:206-253).  The fused kernel
+streams ``pop[n]`` and collides the result in one sweep, so after step n
+the engine holds ``pop[n]`` and defers the stream; ``materialize()`` runs
+the stream-only sweep when ``pop_new[n]`` is wanted.  The values are the
+reference's either way.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from pylabolt_amd import _native as nt
+
+ORIGIN = 63
+ALIGN = 64
+
+_TORCH_DTYPES = {
+    nt.STORE_F64: (torch.float64, torch.float64),
+    nt.STORE_F32: (torch.float32, torch.float32),
+    nt.STORE_F16DEV: (torch.float16, torch.float32),
+    nt.STORE_BF16DEV: (torch.bfloat16, torch.float32),
+}
+
+# D2Q9 directions (reference base/lattice.py:50-59)
+CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+_KIND = {"bounce_back": nt.RULE_BB, "fixed_velocity": nt.RULE_VEL,
+         "fixed_pressure": nt.RULE_PRES, "zero_gradient": nt.RULE_ZG}
+
+
+def geometry(nx, ny):
+    """(pitch, plane) of the pitched layout for an nx x ny interior."""
+    pitch = int(math.ceil((ny + 2) / ALIGN) * ALIGN)
+    plane = (nx + 2) * pitch + ALIGN
+    return pitch, plane
+
+
+def padded_view(flat, q, nx, ny, pitch, plane):
+    """View of plane ``q`` of ``flat`` as the padded (nx+2, ny+2) grid."""
+    base = q * plane + ORIGIN
+    return flat[base:base + (nx + 2) * pitch].reshape(nx + 2, pitch)[:, :ny + 2]
+
+
+class CaseTables:
+    """Host-built per-cell flags, boundary rules and tile classes.
+
+    Built from the reference-shaped case arrays: ``solid`` over the padded
+    grid, the ordered boundary elements and the periodic flags.
+    """
+
+    def __init__(self, nx, ny, solid_pad, elements, x_periodic, y_periodic,
+                 tile_j=256, tile_i=1):
+        self.nx, self.ny = nx, ny
+        self.x_periodic, self.y_periodic = bool(x_periodic), bool(y_periodic)
+        nxp, nyp = nx + 2, ny + 2
+        solid = np.asarray(solid_pad, dtype=bool).reshape(nxp, nyp)
+        ghost = np.zeros((nxp, nyp), dtype=bool)
+        ghost[0, :] = ghost[-1, :] = ghost[:, 0] = ghost[:, -1] = True
+        fluid = ~solid & ~ghost
+        near = np.zeros_like(fluid)
+        for k in range(1, 9):
+            sh = np.zeros_like(solid)
+            # neighbour pulled from: (i - cx, j - cy)
+            src = np.roll(np.roll(solid, CX[k], axis=0), CY[k], axis=1)
+            sh[1:-1, 1:-1] = src[1:-1, 1:-1]
+            near |= sh
+        near &= fluid
+        # boundary rules, last element wins per (node, direction)
+        rules = {}
+        self.elems = []
+        store = near.copy()
+        for e in elements:
+            if e["type"] == "periodic":
+                continue
+            kind = _KIND[e["type"]]
+            eidx = len(self.elems)
+            self.elems.append((kind, e))
+            nrm = [int(v) for v in e["normal"]]
+            for node in np.asarray(e["nodes"], dtype=np.int64):
+                i, j = divmod(int(node), nyp)
+                if not fluid[i, j]:
+                    continue
+                r = rules.setdefault((i, j), [0] * 9)
+                for v in e["inv"]:
+                    r[int(v)] = kind | (eidx << 4)
+                if kind in (nt.RULE_VEL, nt.RULE_PRES):
+                    store[i, j] = True
+                if kind == nt.RULE_PRES:
+                    store[i + nrm[0], j + nrm[1]] = True
+                if kind == nt.RULE_ZG and near[i + nrm[0], j + nrm[1]]:
+                    store[i + nrm[0], j + nrm[1]] = True
+        if len(self.elems) >= 4096:
+            raise ValueError("at most 4095 boundary elements are supported")
+        bc = np.zeros_like(fluid)
+        for (i, j) in rules:
+            bc[i, j] = True
+        store &= fluid
+        flags = (fluid * nt.CELL_FLUID + solid * nt.CELL_SOLID +
+                 near * nt.CELL_NEAR_SOLID + bc * nt.CELL_BC +
+                 store * nt.CELL_STORE_MOM).astype(np.uint8)
+        self.flags_pad = flags
+        self.fluid_pad = fluid
+        keys = sorted(rules.keys())
+        self.bc_nodes = np.zeros((max(1, len(keys)), 10), dtype=np.uint16)
+        self.bc_slot_pad = np.zeros((nxp, nyp), dtype=np.int32)
+        for n, key in enumerate(keys):
+            self.bc_nodes[n, :9] = rules[key]
+            self.bc_slot_pad[key] = n
+        self.n_bc_nodes = len(keys)
+        self.set_tile(tile_j, tile_i)
+
+    def set_tile(self, tile_j, tile_i):
+        """Classify tiles of tile_j x tile_i interior cells: 0 = plain."""
+        if tile_j * tile_i != 256:
+            raise ValueError("tile_j * tile_i must be 256")
+        self.tile_j, self.tile_i = int(tile_j), int(tile_i)
+        nx, ny = self.nx, self.ny
+        inner = self.flags_pad[1:-1, 1:-1]
+        plain = inner == nt.CELL_FLUID
+        if self.x_periodic:
+            plain[0, :] = False
+            plain[-1, :] = False
+        if self.y_periodic:
+            plain[:, 0] = False
+            plain[:, -1] = False
+        ntj = -(-ny // self.tile_j)
+        nti = -(-nx // self.tile_i)
+        padded = np.ones((nti * self.tile_i, ntj * self.tile_j), dtype=bool)
+        padded[:nx, :ny] = plain
+        tiles = padded.reshape(nti, self.tile_i, ntj, self.tile_j).all(axis=(1, 3))
+        self.tile_class = np.where(tiles, 0, 1).astype(np.uint8).ravel()
+        self.ntj, self.nti = ntj, nti
+        return self.tile_class
+
+
+def elements_from_boundary(boundary):
+    """Ordered element dicts from a ``Boundary`` (reference attribute
+    names, base/boundary.py:7-84)."""
+    out = []
+    for e in boundary.boundary_elements:
+        out.append(dict(
+            type=e.type_fluid, nodes=np.asarray(e.boundary_nodes, np.int64),
+            out=[int(v) for v in e.out_list], inv=[int(v) for v in e.inv_list],
+            vector=[float(v) for v in e.vector_fluid],
+            scalar=float(e.scalar_fluid),
+            normal=[int(round(float(v))) for v in e.surface_normals]))
+    return out
+
+
+class D2Q9Engine:
+    """Device state and kernel calls of one lattice on one GPU.
+
+    Parameters mirror the reference's case description: padded ``solid``
+    mask, initial ``density``/``velocity`` (reference layouts), ordered
+    boundary ``elements``, periodic flags, collision ``model`` ("BGK" or
+    "MRT"), ``forcing`` (None, "guo_linear", "guo_second_order"),
+    ``omega``, ``gravity``; plus the storage form ("f64", "f32", "f16dev",
+    "bf16dev") and an optional per-cell ``force_field`` (padded
+    ``[size, 2]``) that replaces ``rho * gravity``.
+    """
+
+    def __init__(self, nx, ny, solid_pad, density, velocity, elements,
+                 x_periodic, y_periodic, model, forcing, omega, gravity,
+                 inv_cs_2, inv_cs_4, weights, float_min, storage="f64",
+                 force_field=None, tile=(256, 1), rates=None, dev_scale=None,
+                 device=None, graph_steps=16):
+        self.lib = nt.lib()
+        self.device = (torch.device(device) if device is not None else
+                       torch.device("cuda", torch.cuda.current_device()))
+        self.nx, self.ny = int(nx), int(ny)
+        self.pitch, self.plane = geometry(self.nx, self.ny)
+        self.storage_name = storage
+        self.storage = nt.STORAGE_CODES[storage]
+        self.raw_dtype, self.ctype = _TORCH_DTYPES[self.storage]
+        self.model = model
+        self.forcing = forcing
+        self.weights = np.asarray(weights, dtype=np.float64)
+        self.tables = CaseTables(self.nx, self.ny, solid_pad, elements,
+                                 x_periodic, y_periodic, tile[0], tile[1])
+        self.stream = torch.cuda.Stream(device=self.device)
+        self._stream_ptr = ctypes.c_void_p(self.stream.cuda_stream)
+        dev = self.device
+        P = self.plane
+        with torch.cuda.stream(self.stream):
+            self.f = [torch.empty(9 * P, dtype=self.raw_dtype, device=dev)
+                      for _ in range(2)]
+            self.mom = [torch.zeros(3 * P, dtype=self.ctype, device=dev)
+                        for _ in range(2)]
+            self.flags = self._upload_plane(self.tables.flags_pad, torch.uint8)
+            self.bc_slot = self._upload_plane(self.tables.bc_slot_pad,
+                                              torch.int32)
+            self.bc_nodes = torch.from_numpy(
+                self.tables.bc_nodes.view(np.int16).copy()).to(dev)
+            elems = (nt.BcElem * max(1, len(self.tables.elems)))()
+            for n, (kind, e) in enumerate(self.tables.elems):
+                elems[n].vec[0], elems[n].vec[1] = e["vector"]
+                elems[n].scalar = e["scalar"]
+                elems[n].normal[0], elems[n].normal[1] = e["normal"]
+                elems[n].type = kind
+            raw = bytes(elems)
+            self.bc_elems = torch.frombuffer(bytearray(raw),
+                                             dtype=torch.uint8).to(dev)
+            self.tile_class = torch.from_numpy(self.tables.tile_class).to(dev)
+            self.force_field = None
+            if force_field is not None:
+                self.force_field = self._pitched_vector(force_field)
+            self.work = torch.zeros(6 * 2048, dtype=torch.float64, device=dev)
+            self.red_out = torch.zeros(8, dtype=torch.float64, device=dev)
+        L = nt.Lattice()
+        L.nx, L.ny, L.pitch, L.origin, L.plane = (self.nx, self.ny, self.pitch,
+                                                 ORIGIN, P)
+        L.storage = self.storage
+        L.collision = nt.MODEL_CODES[model]
+        L.forcing = nt.FORCING_CODES[forcing]
+        L.force_source = nt.FSRC_FIELD if force_field is not None else nt.FSRC_GRAVITY
+        L.x_periodic, L.y_periodic = int(x_periodic), int(y_periodic)
+        L.tile_j, L.tile_i = self.tables.tile_j, self.tables.tile_i
+        L.omega = float(omega)
+        L.inv_cs_2, L.inv_cs_4, L.float_min = (float(inv_cs_2), float(inv_cs_4),
+                                               float(float_min))
+        for k in range(9):
+            L.w[k] = float(self.weights[k])
+        L.gravity[0], L.gravity[1] = float(gravity[0]), float(gravity[1])
+        if rates is None:
+            rates = [1, 1, 1, 1, 1, 1, 1, omega, omega]
+        for k in range(9):
+            L.rates[k] = float(rates[k])
+        if dev_scale is None:
+            dev_scale = [1.0] * 9
+        for k in range(9):
+            L.dev_scale[k] = float(dev_scale[k])
+        L.flags = self.flags.data_ptr()
+        L.tile_class = self.tile_class.data_ptr()
+        if self.tables.n_bc_nodes:
+            L.bc_slot = self.bc_slot.data_ptr()
+            L.bc_nodes = self.bc_nodes.data_ptr()
+        L.bc_elems = self.bc_elems.data_ptr()
+        L.force_field = (self.force_field.data_ptr()
+                         if self.force_field is not None else None)
+        self.L = L
+        self._Lp = ctypes.byref(self.L)
+        self.graph_steps = max(2, int(graph_steps) // 2 * 2)
+        self._graphs = {}
+        self.time_step = 0
+        self._init_density = np.asarray(density, dtype=np.float64)
+        self._init_velocity = np.asarray(velocity, dtype=np.float64)
+        self.initialize()
+
+    # ---- layout helpers -------------------------------------------------
+    def _upload_plane(self, pad, dtype, count=1):
+        host = np.zeros(count * self.plane, dtype=pad.dtype)
+        padded_view(host, 0, self.nx, self.ny, self.pitch, self.plane)[:] = pad
+        return torch.from_numpy(host).to(self.device).view(dtype)
+
+    def _pitched_vector(self, arr_size2):
+        arr = np.asarray(arr_size2, dtype=np.float64).reshape(
+            self.nx + 2, self.ny + 2, 2)
+        host = np.zeros(2 * self.plane, dtype=np.float64)
+        for q in range(2):
+            padded_view(host, q, self.nx, self.ny, self.pitch,
+                        self.plane)[:] = arr[:, :, q]
+        return torch.from_numpy(host).to(self.device, self.ctype)
+
+    def _mom_from_host(self, density, velocity):
+        host = np.zeros(3 * self.plane, dtype=np.float64)
+        nxp, nyp = self.nx + 2, self.ny + 2
+        padded_view(host, 0, self.nx, self.ny, self.pitch, self.plane)[:] = \
+            np.asarray(density).reshape(nxp, nyp)
+        vel = np.asarray(velocity).reshape(nxp, nyp, 2)
+        for q in range(2):
+            padded_view(host, q + 1, self.nx, self.ny, self.pitch,
+                        self.plane)[:] = vel[:, :, q]
+        return torch.from_numpy(host).to(self.device, self.ctype)
+
+    def _grid(self, flat, q):
+        return padded_view(flat, q, self.nx, self.ny, self.pitch, self.plane)
+
+    # ---- lifecycle ------------------------------------------------------
+    def initialize(self):
+        """pop = pop_new = feq(rho0, u0) (equilibrium_kernels.py:38-78)."""
+        with torch.cuda.stream(self.stream):
+            m = self._mom_from_host(self._init_density, self._init_velocity)
+            self.mom[0].copy_(m)
+            self.mom[1].copy_(m)
+        nt.check(self.lib.d2q9_init_eq(self._Lp, nt.ptr(self.f[0]),
+                                       nt.ptr(self.f[1]), nt.ptr(self.mom[0]),
+                                       self._stream_ptr), "d2q9_init_eq")
+        # both buffers hold pop_new[0] == pop[0]
+        self.i_pop = 1
+        self.i_pop_new = 0
+        self.i_mom = 0
+        self.mom_full = True
+        self.time_step = 0
+
+    def close(self):
+        for g in self._graphs.values():
+            self.lib.d2q9_graph_destroy(g)
+        self._graphs = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- kernels --------------------------------------------------------
+    def _collide(self, src, dst, write_mom):
+        nt.check(self.lib.d2q9_collide(
+            self._Lp, nt.ptr(self.f[src]), nt.ptr(self.f[dst]),
+            nt.ptr(self.mom[self.i_mom]), nt.ptr(self.mom[1 - self.i_mom]),
+            int(write_mom), self._stream_ptr), "d2q9_collide")
+        self.i_mom = 1 - self.i_mom
+
+    def _fused(self, src, dst, write_mom):
+        nt.check(self.lib.d2q9_fused_step(
+            self._Lp, nt.ptr(self.f[src]), nt.ptr(self.f[dst]),
+            nt.ptr(self.mom[self.i_mom]), nt.ptr(self.mom[1 - self.i_mom]),
+            int(write_mom), self._stream_ptr), "d2q9_fused_step")
+        self.i_mom = 1 - self.i_mom
+
+    def step(self, write_moments=False):
+        """Advance one reference time step."""
+        if self.i_pop_new is not None:
+            # pop_new is in memory: moments + collision only
+            src = self.i_pop_new
+            dst = 1 - src
+            self._collide(src, dst, write_moments)
+        else:
+            src = self.i_pop
+            dst = 1 - src
+            self._fused(src, dst, write_moments)
+        self.i_pop = dst
+        self.i_pop_new = None
+        self.mom_full = bool(write_moments)
+        self.time_step += 1
+
+    def advance(self, n, write_moments=True, use_graph=True):
+        """Advance ``n`` steps; the last one stores rho/u on every fluid
+        cell when ``write_moments``.  Full ``graph_steps`` chunks of fused
+        steps replay a captured HIP graph."""
+        n = int(n)
+        if n <= 0:
+            return
+        if self.i_pop_new is not None:
+            self.step(write_moments and n == 1)
+            n -= 1
+        # full chunks of fused steps replay a captured graph; the last step
+        # stays eager so that it can store the moments
+        if use_graph:
+            while n - 1 >= self.graph_steps:
+                g = self._graph_for_state()
+                nt.check(self.lib.d2q9_graph_launch(g, self._stream_ptr),
+                         "graph launch")
+                self.time_step += self.graph_steps
+                n -= self.graph_steps
+        while n > 0:
+            self.step(write_moments and n == 1)
+            n -= 1
+
+    def _graph_for_state(self):
+        key = (self.i_pop, self.i_mom)
+        g = self._graphs.get(key)
+        if g is None:
+            saved_mom = self.i_mom
+            g = self._graph_capture(self.i_pop, self.i_mom)
+            self.i_mom = saved_mom
+            self._graphs[key] = g
+        # graph_steps is even: buffers return to the same roles
+        self.mom_full = False
+        return g
+
+    def _graph_capture(self, start, im):
+        nt.check(self.lib.d2q9_graph_begin(self._stream_ptr), "graph begin")
+        err = None
+        src = start
+        for _ in range(self.graph_steps):
+            rc = self.lib.d2q9_fused_step(
+                self._Lp, nt.ptr(self.f[src]), nt.ptr(self.f[1 - src]),
+                nt.ptr(self.mom[im]), nt.ptr(self.mom[1 - im]), 0,
+                self._stream_ptr)
+            if rc != 0 and err is None:
+                err = self.lib.d2q9_last_error().decode()
+            src, im = 1 - src, 1 - im
+        handle = ctypes.c_void_p()
+        rc_end = self.lib.d2q9_graph_end(self._stream_ptr, ctypes.byref(handle))
+        if err is not None:
+            raise nt.NativeError("graph capture: " + err)
+        nt.check(rc_end, "graph end")
+        return handle
+
+    def materialize(self):
+        """Run the deferred stream + boundary sweep: pop_new of the current
+        step into the spare buffer."""
+        if self.i_pop_new is not None:
+            return self.i_pop_new
+        dst = 1 - self.i_pop
+        nt.check(self.lib.d2q9_stream_bc(
+            self._Lp, nt.ptr(self.f[self.i_pop]), nt.ptr(self.f[dst]),
+            nt.ptr(self.mom[self.i_mom]), self._stream_ptr), "d2q9_stream_bc")
+        self.i_pop_new = dst
+        return dst
+
+    def synchronize(self):
+        self.stream.synchronize()
+
+    # ---- reductions -----------------------------------------------------
+    def mass(self, which="pop"):
+        """FP64 sum of the populations over fluid cells."""
+        idx = self.i_pop if which == "pop" else self.materialize()
+        nt.check(self.lib.d2q9_mass_fp64(
+            self._Lp, nt.ptr(self.f[idx]), nt.ptr(self.work),
+            nt.ptr(self.red_out), self._stream_ptr), "d2q9_mass_fp64")
+        self.synchronize()
+        return float(self.red_out[0].item())
+
+    def residue_sums(self, mom_old):
+        """Six FP64 residue sums of the current moments against
+        ``mom_old`` (updated in place)."""
+        if not self.mom_full:
+            raise RuntimeError("moments of this step were not stored")
+        nt.check(self.lib.d2q9_residues(
+            self._Lp, nt.ptr(self.mom[self.i_mom]), nt.ptr(mom_old),
+            nt.ptr(self.work), nt.ptr(self.red_out), self._stream_ptr),
+            "d2q9_residues")
+        self.synchronize()
+        return self.red_out[:6].cpu().numpy().copy()
+
+    # ---- host views (reference layouts) -----------------------------------
+    def _decode_aos(self, idx):
+        """Populations of buffer ``idx`` as a float64 numpy [size, 9]."""
+        with torch.cuda.stream(self.stream):
+            planes = []
+            for k in range(9):
+                g = self._grid(self.f[idx], k).to(torch.float64)
+                if self.storage in (nt.STORE_F16DEV, nt.STORE_BF16DEV):
+                    g = self.weights[k] + g / float(self.L.dev_scale[k])
+                planes.append(g)
+            aos = torch.stack(planes, dim=-1).reshape(-1, 9).cpu().numpy()
+        if self.storage in (nt.STORE_F16DEV, nt.STORE_BF16DEV):
+            aos[~self.tables.fluid_pad.ravel()] = 0.0
+        return aos
+
+    def pop(self):
+        """Reference ``pop_fluid`` (post-collision) as float64 [size, 9]."""
+        return self._decode_aos(self.i_pop)
+
+    def pop_new(self):
+        """Reference ``pop_fluid_new`` as float64 [size, 9]."""
+        return self._decode_aos(self.materialize())
+
+    def moments_host(self):
+        """(density [size], velocity [size, 2]) of the current step."""
+        if not self.mom_full:
+            raise RuntimeError(
+                "moments of this step were not stored: advance with "
+                "write_moments=True")
+        with torch.cuda.stream(self.stream):
+            m = self.mom[self.i_mom]
+            rho = self._grid(m, 0).to(torch.float64).reshape(-1).cpu().numpy()
+            ux = self._grid(m, 1).to(torch.float64).reshape(-1).cpu().numpy()
+            uy = self._grid(m, 2).to(torch.float64).reshape(-1).cpu().numpy()
+        return rho, np.stack([ux, uy], axis=-1)
