@@ -1,0 +1,261 @@
+"""Benchmark: MLUPS and HBM roofline of the fused D2Q9 sweep on one MI355X.
+
+Workload (default): flow past a cylinder, D2Q9 BGK, 8192 x 2048 interior
+cells, cylinder D=128, fixed-pressure inlet/outlet, zero-gradient top and
+bottom, FP64 populations -- the configuration the metric is quoted on.
+Synthetic initial state (rest, rho = 1), as in the reference's case files.
+
+One "step" is one reference time step (stream + boundaries + moments +
+collision) over the whole lattice: one fused kernel launch, replayed from a
+HIP graph.  MLUPS = Nx * Ny * steps / t / 1e6.  The kernel time comes from
+HIP events recorded on the engine's own stream around the timed steps.
+
+Roofline: algorithmic traffic per cell update is 18 populations (9 read,
+9 written): 144 B in FP64, 72 B in FP32, 36 B for 16-bit storage.  Peak is
+MI355X's 8 TB/s HBM3E.  ``traffic`` is the measured HBM bytes per launch
+from rocprofv3 FETCH_SIZE/WRITE_SIZE counters (profiles/), when present.
+
+The CPU baseline is the C++/OpenMP restatement of the reference's
+multi-pass step (oracle/d2q9_ref.cpp) on the same workload for a bounded
+number of steps, on the host cores.
+
+Also reported (``forms``): FP32 and 16-bit-storage runs of the same
+workload with their roofline fractions and velocity error against the
+FP64 run, and a plain copy kernel's bandwidth on the same device.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BYTES_PER_CELL = {"f64": 144, "f32": 72, "f16dev": 36, "bf16dev": 36}
+DTYPES = {"f64": "f64", "f32": "f32", "f16dev": "f32-compute/f16-store",
+          "bf16dev": "f32-compute/bf16-store"}
+PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_workload(name, nx, ny, steps):
+    from pylabolt_amd import cases
+    if name == "cylinder":
+        return cases.cylinder(nx or 8192, ny or 2048, steps=steps)
+    if name == "poiseuille":
+        return cases.poiseuille_channel(nx or 4096, ny or 1024, steps=steps)
+    if name == "cavity":
+        return cases.lid_driven_cavity(nx or 128, ny or 128, steps=steps)
+    raise ValueError(name)
+
+
+def time_engine(solver, steps, warmup):
+    import torch
+    eng = solver.engine
+    eng.advance(warmup, write_moments=False)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(eng.stream)
+    eng.advance(steps, write_moments=False)
+    stop.record(eng.stream)
+    stop.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(stop)
+    return ms, wall
+
+
+def copy_bandwidth(nbytes=2 << 30, reps=20):
+    """Bandwidth of the library's plain copy kernel (read + write bytes)."""
+    import ctypes
+    import torch
+    from pylabolt_amd import _native as nt
+    lib = nt.lib()
+    s = torch.cuda.Stream()
+    sp = ctypes.c_void_p(s.cuda_stream)
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        nt.check(lib.d2q9_copy(nt.ptr(b), nt.ptr(a), nbytes, sp), "copy")
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        nt.check(lib.d2q9_copy(nt.ptr(b), nt.ptr(a), nbytes, sp), "copy")
+    e1.record(s)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(sim, target_s=15.0, max_steps=200):
+    """C++/OpenMP reference restatement on this host, bounded sample."""
+    from oracle import ref_lib
+    from tests.helpers import oracle_case
+    case = oracle_case(sim)
+    r = ref_lib.RefRunner(case)
+    r.initialize()
+    threads = ref_lib.max_threads()
+    t0 = time.perf_counter()
+    r.run(2)
+    per = (time.perf_counter() - t0) / 2
+    n = int(max(1, min(max_steps, target_s / max(per, 1e-9))))
+    t0 = time.perf_counter()
+    r.run(n)
+    dt = time.perf_counter() - t0
+    nx, ny = sim.mesh_dict["grid"]
+    return {"value": nx * ny * n / dt / 1e6, "unit": "MLUPS",
+            "cores": threads, "kind": "port",
+            "sample": f"{n} steps of the same {nx}x{ny} workload after 2 "
+                      f"warm-up steps, OpenMP threads={threads}"}
+
+
+def load_pmc(workload, storage):
+    path = os.path.join(ROOT, "profiles", f"pmc_{workload}_{storage}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def run_form(sim, storage, steps, warmup, tile):
+    from pylabolt_amd.solvers.fluidLB import Solver
+    solver = Solver(simulation=sim, storage=storage, tile=tile,
+                    graph_steps=32, verbose=False)
+    solver.set_backend()
+    ms, wall = time_engine(solver, steps, warmup)
+    return solver, ms, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="cylinder")
+    ap.add_argument("--nx", type=int, default=0)
+    ap.add_argument("--ny", type=int, default=0)
+    ap.add_argument("--storage", default="f64")
+    ap.add_argument("--tile", default="64x4", help="tile rows: must be 4")
+    ap.add_argument("--forms", default="f32,f16dev,bf16dev",
+                    help="extra storage forms to time ('' for none)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-copy", action="store_true")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="a short run for rocprofv3: main kernel + copy")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must "
+                          "be 1 (multi-GPU sharding is a design note only)"}))
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    tile = tuple(int(v) for v in args.tile.split("x"))
+    sim = make_workload(args.workload, args.nx, args.ny,
+                        args.steps + args.warmup)
+    nx, ny = sim.mesh_dict["grid"]
+    cells = nx * ny
+
+    if args.profile_run:
+        solver, ms, wall = run_form(sim, args.storage, args.steps,
+                                    args.warmup, tile)
+        bw = copy_bandwidth(1 << 30, reps=5)
+        log(f"profile run: {ms / args.steps * 1e3:.1f} us/step, copy {bw:.0f} GB/s")
+        return
+
+    log(f"workload {args.workload} {nx}x{ny} storage={args.storage} "
+        f"tile={tile} steps={args.steps} warmup={args.warmup}")
+    solver, ms, wall = run_form(sim, args.storage, args.steps, args.warmup,
+                                tile)
+    ms_step = ms / args.steps
+    mlups = cells / (ms_step * 1e-3) / 1e6
+    bpc = BYTES_PER_CELL[args.storage]
+    achieved = bpc * cells / (ms_step * 1e-3) / 1e9
+    log(f"{args.storage}: {ms_step * 1e3:.1f} us/step  {mlups:.0f} MLUPS  "
+        f"{achieved:.0f} GB/s ({achieved / PEAK_GBS:.3f} of peak), wall "
+        f"{wall:.2f}s")
+    ref_rho, ref_vel = None, None
+    solver.engine.advance(1, write_moments=True)
+    ref_rho, ref_vel = solver.engine.moments_host()
+    fluid = solver.engine.tables.fluid_pad.ravel()
+    steps_done = solver.engine.time_step
+    del solver
+    torch.cuda.empty_cache()
+
+    forms = {args.storage: {"mlups": mlups, "us_per_step": ms_step * 1e3,
+                            "gbs": achieved, "frac": achieved / PEAK_GBS}}
+    for form in [f for f in args.forms.split(",") if f and f != args.storage]:
+        s2, ms2, _ = run_form(sim, form, args.steps, args.warmup, tile)
+        st2 = ms2 / args.steps
+        m2 = cells / (st2 * 1e-3) / 1e6
+        a2 = BYTES_PER_CELL[form] * cells / (st2 * 1e-3) / 1e9
+        s2.engine.advance(steps_done - s2.engine.time_step, write_moments=True)
+        _, vel2 = s2.engine.moments_host()
+        du = vel2[fluid] - ref_vel[fluid]
+        l2 = float(np.sqrt((du ** 2).sum() / max((ref_vel[fluid] ** 2).sum(),
+                                                 1e-300)))
+        forms[form] = {"mlups": m2, "us_per_step": st2 * 1e3, "gbs": a2,
+                       "frac": a2 / PEAK_GBS,
+                       "speedup_vs_" + args.storage: m2 / mlups,
+                       "u_rel_l2_vs_" + args.storage: l2,
+                       "steps_compared": steps_done}
+        log(f"{form}: {st2 * 1e3:.1f} us/step {m2:.0f} MLUPS {a2:.0f} GB/s "
+            f"u L2 rel err {l2:.2e}")
+        del s2
+        torch.cuda.empty_cache()
+
+    copy_gbs = None if args.no_copy else copy_bandwidth()
+    if copy_gbs:
+        log(f"copy kernel: {copy_gbs:.0f} GB/s")
+    pmc = load_pmc(args.workload, args.storage)
+    traffic = pmc["hbm_bytes_per_launch"] if pmc else None
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(sim)
+        log(f"cpu: {cpu['value']:.1f} MLUPS on {cpu['cores']} threads")
+
+    out = {
+        "metric": "MLUPS + % HBM roofline (FP64 vs FP32/FP16-store); "
+                  "adjoint/forward cost ratio",
+        "value": mlups,
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPES[args.storage],
+        "data": "synthetic (seeded case file, fluid at rest, rho=1)",
+        "config": {"workload": f"{args.workload} D2Q9 BGK {nx}x{ny}",
+                   "storage": args.storage, "cells": cells,
+                   "tile": f"{tile[0]}x{tile[1]}", "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_GBS,
+                     "traffic": traffic,
+                     "bytes_per_cell_update": bpc},
+        "cpu_baseline": cpu,
+        "forms": forms,
+        "copy_kernel_gbs": copy_gbs,
+    }
+    if copy_gbs:
+        out["roofline"]["frac_of_copy"] = achieved / copy_gbs
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -47,7 +47,7 @@ extern "C" {
 /* storage of the populations in HBM */
 enum {
   D2Q9_STORE_F64 = 0,     /* double, compute in double (the parity path) */
-  D2Q9_STORE_F32 = 1,     /* float, compute in float */
+  D2Q9_STORE_F32 = 1,     /* float of (f_i - w_i) * dev_scale_i, compute float */
   D2Q9_STORE_F16DEV = 2,  /* half of (f_i - w_i) * dev_scale_i, compute float */
   D2Q9_STORE_BF16DEV = 3  /* bfloat16 of the same, compute float */
 };
@@ -94,24 +94,58 @@ typedef struct d2q9_bc_node {
 } d2q9_bc_node;
 
 /*
- * One lattice.  Padded cell (i, j), 0 <= i <= nx+1, 0 <= j <= ny+1, lives at
- * element origin + i * pitch + j of every plane; plane q of the populations
- * starts at q * plane.  The engine chooses origin = 63 and pitch a multiple
- * of 64 so that every interior row starts on a 64-element boundary.
+ * One lattice, padded cells (i, j), 0 <= i <= nx+1, 0 <= j <= ny+1.
+ *
+ * Populations (AoSoA): direction k of cell (i, j) is element
+ *   ((i * nbj + jj / B) * 9 + k) * B + jj % B,  jj = j + B - 1,
+ * with B = 64 * cpl cells per block; cpl = 1 (FP64), 2 (FP32), 4 (16-bit)
+ * so that one lane of a wave moves 8 bytes per direction.  A buffer holds
+ * pop_len = (nx + 2) * nbj * 9 * B elements, nbj >= (ny + B) / B + 1.
+ *
+ * Per-cell side arrays (flags, bc_slot, moments [rho|ux|uy], force field
+ * [fx|fy]) are pitched planes: cell (i, j) is element
+ * origin + i * pitch + j of each plane of `plane` elements.
  */
+/* Single-precision copies of the scalar constants, filled by the caller
+ * from the double fields (the FP32-compute kernels read these). */
+typedef struct d2q9_fconst {
+  float w[9], rates[9], scale[9], inv_scale[9];
+  float omega, inv_cs_2, inv_cs_4, float_min;
+  float gravity[2];
+} d2q9_fconst;
+
 typedef struct d2q9_lattice {
   int32_t nx, ny, pitch, origin;
   int64_t plane;
+  int32_t cpl, nbj;
+  int64_t pop_len;
   int32_t storage, collision, forcing, force_source;
   int32_t x_periodic, y_periodic;
-  int32_t tile_j, tile_i;       /* launch tile: tile_j * tile_i == 256 */
+  int32_t tile_j, tile_i;       /* launch tile: tile_j == 64 * cpl, tile_i == 4 */
   double omega, inv_cs_2, inv_cs_4, float_min;
   double w[9];
   double gravity[2];
   double rates[9];              /* MRT relaxation rates */
   double dev_scale[9];          /* 16-bit storage scale per direction */
+  d2q9_fconst cf;
   const uint8_t* flags;         /* [plane] */
-  const uint8_t* tile_class;    /* [tiles]: 0 = plain tile, or NULL */
+  /* Tile lists: tiles[0 .. n_bulk) are bulk tiles (every cell fluid and
+   * inside the lattice), tiles[n_bulk .. n_bulk + n_general) the others.
+   * Tile t covers rows 1 + (t / ntj) * tile_i .. and cells
+   * 1 + (t % ntj) * tile_j .., ntj = ceil(ny / tile_j).  NULL: every tile
+   * is handled as general.  The bulk sweep treats every cell as plain
+   * interior fluid; fix_cells lists (as origin + i * pitch + j) the cells
+   * of bulk tiles that are not -- solid neighbour, boundary rule, stored
+   * moment, periodic edge -- and a second pass recomputes them. */
+  const int32_t* tiles;
+  int32_t n_bulk, n_general;
+  const int32_t* fix_cells;
+  int32_t n_fix, pad_fix;
+  /* Special-cell bits of the bulk tiles: for the tile at position p of the
+   * bulk list and its row r, words [(p * tile_i + r) * cpl, + cpl) hold one
+   * bit per cell of the block (bit b of word w: cell 1 + tj*tile_j + 64w + b).
+   * The bulk sweep does not store those cells; the fix-up pass does. */
+  const uint64_t* bulk_masks;
   const int32_t* bc_slot;       /* [plane], read where D2Q9_CELL_BC */
   const d2q9_bc_node* bc_nodes;
   const d2q9_bc_elem* bc_elems;
@@ -133,6 +167,14 @@ int d2q9_collide(const d2q9_lattice* L, const void* f_src, void* f_dst,
 /* Stream + periodic wrap + boundary rules only: f_dst = pop_new. */
 int d2q9_stream_bc(const d2q9_lattice* L, const void* f_src, void* f_dst,
                    const void* mom_src, void* stream);
+
+/* The sweeps below launch the bulk tiles on `stream` and the general tiles
+ * plus fix-up cells on an auxiliary stream of the same device, forked from
+ * and joined back to `stream` with events (both passes write disjoint
+ * cells), so the two overlap; the library creates that stream and its two
+ * events on first use per device and keeps them for the process.  Work is
+ * ordered on `stream` exactly as if it ran there alone, including under
+ * HIP graph capture. */
 
 /* Fused: stream + boundaries of step n, moments + collision of step n+1. */
 int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
@@ -166,6 +208,14 @@ int d2q9_graph_destroy(void* graph_exec);
 
 /* Text of the last error on this thread. */
 const char* d2q9_last_error(void);
+
+/* Struct layout for binding checks: sizeof(d2q9_lattice), sizeof of the two
+ * boundary structs, then offsetof(d2q9_lattice, f) for f in plane, cpl,
+ * pop_len, storage, tile_j, omega, w, gravity, rates, dev_scale, cf,
+ * flags, tiles, n_bulk, fix_cells, n_fix, bulk_masks, bc_slot, bc_nodes,
+ * bc_elems, force_field.  Returns the count written (24) or D2Q9_E_ARG if n
+ * is too small.  No GPU needed. */
+int d2q9_abi_layout(int64_t* out, int n);
 
 #ifdef __cplusplus
 }

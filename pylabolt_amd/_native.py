@@ -38,12 +38,23 @@ class BcNode(ctypes.Structure):
     _fields_ = [("rule", ctypes.c_uint16 * 9), ("pad", ctypes.c_uint16)]
 
 
+class FConst(ctypes.Structure):
+    """Mirror of ``d2q9_fconst``."""
+    _fields_ = [("w", ctypes.c_float * 9), ("rates", ctypes.c_float * 9),
+                ("scale", ctypes.c_float * 9), ("inv_scale", ctypes.c_float * 9),
+                ("omega", ctypes.c_float), ("inv_cs_2", ctypes.c_float),
+                ("inv_cs_4", ctypes.c_float), ("float_min", ctypes.c_float),
+                ("gravity", ctypes.c_float * 2)]
+
+
 class Lattice(ctypes.Structure):
     """Mirror of ``d2q9_lattice``."""
     _fields_ = [
         ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
         ("pitch", ctypes.c_int32), ("origin", ctypes.c_int32),
         ("plane", ctypes.c_int64),
+        ("cpl", ctypes.c_int32), ("nbj", ctypes.c_int32),
+        ("pop_len", ctypes.c_int64),
         ("storage", ctypes.c_int32), ("collision", ctypes.c_int32),
         ("forcing", ctypes.c_int32), ("force_source", ctypes.c_int32),
         ("x_periodic", ctypes.c_int32), ("y_periodic", ctypes.c_int32),
@@ -52,7 +63,12 @@ class Lattice(ctypes.Structure):
         ("inv_cs_4", ctypes.c_double), ("float_min", ctypes.c_double),
         ("w", ctypes.c_double * 9), ("gravity", ctypes.c_double * 2),
         ("rates", ctypes.c_double * 9), ("dev_scale", ctypes.c_double * 9),
-        ("flags", ctypes.c_void_p), ("tile_class", ctypes.c_void_p),
+        ("cf", FConst),
+        ("flags", ctypes.c_void_p), ("tiles", ctypes.c_void_p),
+        ("n_bulk", ctypes.c_int32), ("n_general", ctypes.c_int32),
+        ("fix_cells", ctypes.c_void_p),
+        ("n_fix", ctypes.c_int32), ("pad_fix", ctypes.c_int32),
+        ("bulk_masks", ctypes.c_void_p),
         ("bc_slot", ctypes.c_void_p), ("bc_nodes", ctypes.c_void_p),
         ("bc_elems", ctypes.c_void_p), ("force_field", ctypes.c_void_p),
     ]
@@ -78,7 +94,28 @@ SIGNATURES = {
     "d2q9_graph_launch": (ctypes.c_int, [_P, _P]),
     "d2q9_graph_destroy": (ctypes.c_int, [_P]),
     "d2q9_last_error": (ctypes.c_char_p, []),
+    "d2q9_abi_layout": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.c_int]),
 }
+
+# fields whose offsets d2q9_abi_layout reports, in its order
+ABI_FIELDS = ["plane", "cpl", "pop_len", "storage", "tile_j", "omega", "w",
+              "gravity", "rates", "dev_scale", "cf", "flags", "tiles",
+              "n_bulk", "fix_cells", "n_fix", "bulk_masks", "bc_slot",
+              "bc_nodes",
+              "bc_elems", "force_field"]
+
+
+def abi_mismatches(handle):
+    """Compare the ctypes mirror with the library's own struct layout."""
+    out = (ctypes.c_int64 * 32)()
+    n = handle.d2q9_abi_layout(out, 32)
+    got = list(out[:n])
+    want = [ctypes.sizeof(Lattice), ctypes.sizeof(BcElem),
+            ctypes.sizeof(BcNode)]
+    want += [getattr(Lattice, f).offset for f in ABI_FIELDS]
+    return [(i, g, w) for i, (g, w) in enumerate(zip(got, want)) if g != w] \
+        + ([("count", n, len(want))] if n != len(want) else [])
 
 _lib = None
 
@@ -98,6 +135,9 @@ def load_library(path=LIB_PATH):
         fn = getattr(handle, name)
         fn.restype = res
         fn.argtypes = args
+    bad = abi_mismatches(handle)
+    if bad:
+        raise NativeError(f"d2q9_lattice layout differs from the library: {bad}")
     return handle
 
 

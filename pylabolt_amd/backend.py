@@ -13,7 +13,7 @@ from pylabolt_amd.engine import D2Q9Engine, elements_from_boundary
 
 class Backend:
     def __init__(self, comm, state, backend_type="gpu", n_threads=None,
-                 storage="f64", tile=(256, 1), device=None, graph_steps=16):
+                 storage="f64", tile=(64, 4), device=None, graph_steps=16):
         if backend_type != "gpu":
             raise ValueError("Unsupported backend: " + str(backend_type) +
                              " (this engine runs on the GPU only)")

@@ -3,8 +3,8 @@
 // Shared by the single-lattice sweep (d2q9_step.hip), the batched persistent
 // sweep (d2q9_batched.hip) and the adjoint (d2q9_adjoint.hip).
 //
-// Arithmetic follows the reference kernels operation by operation, in the
-// association Python gives them (left to right), so that the FP64 path
+// FP64 arithmetic follows the reference kernels operation by operation, in
+// the association Python gives them (left to right), so that the FP64 path
 // reproduces the reference bit for bit when compiled with
 // -ffp-contract=off:
 //   equilibrium  pylabolt/parallel/cpu/equilibrium_kernels.py:28-35
@@ -13,8 +13,22 @@
 //   pull stream  pylabolt/parallel/cpu/streaming_kernels.py:29-47
 //   boundaries   pylabolt/parallel/cpu/fluid_boundary_kernels.py:5-152
 // The MRT operator, zero-gradient rule and per-cell force source have no
-// reference kernel; their operation order is the one oracle/d2q9_oracle.py
-// defines.
+// reference kernel; their FP64 operation order is the one
+// oracle/d2q9_oracle.py defines.
+//
+// The reduced-precision forms (FP32 compute, FP32/FP16/BF16 storage) hold
+// and store deviations d_k = f_k - w_k and compute moments, equilibrium and
+// collision in deviation form, which keeps the small velocity-carrying part
+// of each population out of the rounding of the O(w_k) rest value.
+//
+// Population layout (AoSoA): padded cell (i, j), direction k lives at
+//   ((i * nbj + jj / B) * 9 + k) * B + jj % B,   jj = j + B - 1,
+// B = 64 * cpl cells per block (one wave covers one block of one row: each
+// lane owns cpl consecutive cells), so a wave's loads and stores of one
+// direction are one contiguous 512 B (FP64) segment and the nine
+// directions of a block are adjacent.  Per-cell side arrays (flags, boundary
+// slots, moments, force field) use the pitched layout
+//   origin + i * pitch + j.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,42 +56,97 @@ __device__ constexpr int MM[9][9] = {
 __device__ constexpr int ROWNORM[9] = {9, 36, 36, 6, 12, 6, 12, 4, 4};
 
 // ---------------------------------------------------------------------------
+// Scalar constants in the compute type, loaded once per thread from the
+// descriptor (double fields for FP64, the float mirror otherwise).
+// ---------------------------------------------------------------------------
+template <typename T> struct Consts {
+  T w[9], rates[9], scale[9], iscale[9];
+  T om, ics2, ics4, fmin, gx, gy;
+};
+
+template <typename T>
+__device__ __forceinline__ Consts<T> load_consts(const d2q9_lattice& L);
+
+template <>
+__device__ __forceinline__ Consts<double> load_consts<double>(const d2q9_lattice& L) {
+  Consts<double> K;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    K.w[k] = L.w[k];
+    K.rates[k] = L.rates[k];
+    K.scale[k] = 1.0;
+    K.iscale[k] = 1.0;
+  }
+  K.om = L.omega;
+  K.ics2 = L.inv_cs_2;
+  K.ics4 = L.inv_cs_4;
+  K.fmin = L.float_min;
+  K.gx = L.gravity[0];
+  K.gy = L.gravity[1];
+  return K;
+}
+
+template <>
+__device__ __forceinline__ Consts<float> load_consts<float>(const d2q9_lattice& L) {
+  Consts<float> K;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    K.w[k] = L.cf.w[k];
+    K.rates[k] = L.cf.rates[k];
+    K.scale[k] = L.cf.scale[k];
+    K.iscale[k] = L.cf.inv_scale[k];
+  }
+  K.om = L.cf.omega;
+  K.ics2 = L.cf.inv_cs_2;
+  K.ics4 = L.cf.inv_cs_4;
+  K.fmin = L.cf.float_min;
+  K.gx = L.cf.gravity[0];
+  K.gy = L.cf.gravity[1];
+  return K;
+}
+
+// ---------------------------------------------------------------------------
 // Storage codecs: raw HBM element <-> compute value.
+//   dev == false: the value is f_k (FP64 reference path)
+//   dev == true:  the value is d_k = f_k - w_k; raw = d_k * scale_k
+// CPL: cells per lane, chosen so that one lane moves 8 bytes per direction.
 // ---------------------------------------------------------------------------
 template <int S> struct Codec;
 
 template <> struct Codec<D2Q9_STORE_F64> {
   using raw = double;
   using T = double;
-  static __device__ __forceinline__ T dec(raw r, int, const d2q9_lattice&) { return r; }
-  static __device__ __forceinline__ raw enc(T v, int, const d2q9_lattice&) { return v; }
-  static __device__ __forceinline__ raw zero(int, const d2q9_lattice&) { return 0.0; }
-  static __device__ __forceinline__ double dec64(raw r, int, const d2q9_lattice&) { return r; }
+  static constexpr bool dev = false;
+  static constexpr int CPL = 1;
+  static __device__ __forceinline__ T dec(raw r, int, const Consts<T>&) { return r; }
+  static __device__ __forceinline__ raw enc(T v, int, const Consts<T>&) { return v; }
+  static __device__ __forceinline__ double f64(raw r, int, const d2q9_lattice&) { return r; }
 };
 
 template <> struct Codec<D2Q9_STORE_F32> {
   using raw = float;
   using T = float;
-  static __device__ __forceinline__ T dec(raw r, int, const d2q9_lattice&) { return r; }
-  static __device__ __forceinline__ raw enc(T v, int, const d2q9_lattice&) { return v; }
-  static __device__ __forceinline__ raw zero(int, const d2q9_lattice&) { return 0.0f; }
-  static __device__ __forceinline__ double dec64(raw r, int, const d2q9_lattice&) { return (double)r; }
+  static constexpr bool dev = true;
+  static constexpr int CPL = 2;
+  static __device__ __forceinline__ T dec(raw r, int k, const Consts<T>& K) { return r * K.iscale[k]; }
+  static __device__ __forceinline__ raw enc(T v, int k, const Consts<T>& K) { return v * K.scale[k]; }
+  static __device__ __forceinline__ double f64(raw r, int k, const d2q9_lattice& L) {
+    return L.w[k] + (double)r / L.dev_scale[k];
+  }
 };
 
-// 16-bit forms store d = (f_k - w_k) * s_k; f_k = w_k + d / s_k.
 template <> struct Codec<D2Q9_STORE_F16DEV> {
   using raw = __half;
   using T = float;
-  static __device__ __forceinline__ T dec(raw r, int k, const d2q9_lattice& L) {
-    return (float)L.w[k] + __half2float(r) * (float)(1.0 / L.dev_scale[k]);
+  static constexpr bool dev = true;
+  static constexpr int CPL = 4;
+  static __device__ __forceinline__ T dec(raw r, int k, const Consts<T>& K) {
+    return __half2float(r) * K.iscale[k];
   }
-  static __device__ __forceinline__ raw enc(T v, int k, const d2q9_lattice& L) {
-    return __float2half_rn((v - (float)L.w[k]) * (float)L.dev_scale[k]);
+  static __device__ __forceinline__ raw enc(T v, int k, const Consts<T>& K) {
+    return __float2half_rn(v * K.scale[k]);
   }
-  static __device__ __forceinline__ raw zero(int k, const d2q9_lattice& L) {
-    return __float2half_rn((0.0f - (float)L.w[k]) * (float)L.dev_scale[k]);
-  }
-  static __device__ __forceinline__ double dec64(raw r, int k, const d2q9_lattice& L) {
+  static __device__ __forceinline__ double f64(raw r, int k, const d2q9_lattice& L) {
     return L.w[k] + (double)__half2float(r) / L.dev_scale[k];
   }
 };
@@ -85,29 +154,58 @@ template <> struct Codec<D2Q9_STORE_F16DEV> {
 template <> struct Codec<D2Q9_STORE_BF16DEV> {
   using raw = __hip_bfloat16;
   using T = float;
-  static __device__ __forceinline__ T dec(raw r, int k, const d2q9_lattice& L) {
-    return (float)L.w[k] + __bfloat162float(r) * (float)(1.0 / L.dev_scale[k]);
+  static constexpr bool dev = true;
+  static constexpr int CPL = 4;
+  static __device__ __forceinline__ T dec(raw r, int k, const Consts<T>& K) {
+    return __bfloat162float(r) * K.iscale[k];
   }
-  static __device__ __forceinline__ raw enc(T v, int k, const d2q9_lattice& L) {
-    return __float2bfloat16((v - (float)L.w[k]) * (float)L.dev_scale[k]);
+  static __device__ __forceinline__ raw enc(T v, int k, const Consts<T>& K) {
+    return __float2bfloat16(v * K.scale[k]);
   }
-  static __device__ __forceinline__ raw zero(int k, const d2q9_lattice& L) {
-    return __float2bfloat16((0.0f - (float)L.w[k]) * (float)L.dev_scale[k]);
-  }
-  static __device__ __forceinline__ double dec64(raw r, int k, const d2q9_lattice& L) {
+  static __device__ __forceinline__ double f64(raw r, int k, const d2q9_lattice& L) {
     return L.w[k] + (double)__bfloat162float(r) / L.dev_scale[k];
   }
 };
 
+// Encoded value of a zero population (f = 0): ghost and solid cells.
+template <int S>
+__device__ __forceinline__ typename Codec<S>::raw enc_zero(int k, const Consts<typename Codec<S>::T>& K) {
+  using T = typename Codec<S>::T;
+  if constexpr (Codec<S>::dev) return Codec<S>::enc(-K.w[k], k, K);
+  else return Codec<S>::enc((T)0, k, K);
+}
+
 // ---------------------------------------------------------------------------
-// Equilibrium and Guo source, reference association.
+// Layout.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t cell_index(const d2q9_lattice& L, int i, int j) {
+  return (int64_t)L.origin + (int64_t)i * L.pitch + j;
+}
+
+template <int B>
+__device__ __forceinline__ int64_t pop_index(const d2q9_lattice& L, int i, int j, int k) {
+  const int jj = j + B - 1;
+  return (((int64_t)i * L.nbj + jj / B) * 9 + k) * B + (jj % B);
+}
+
+// ---------------------------------------------------------------------------
+// Equilibrium and Guo source.
+// ---------------------------------------------------------------------------
+// Reference form: feq_k = (w rho) (((1 + ics2 cu) + ((0.5 ics4) cu) cu) - (0.5 ics2) u2)
 template <typename T>
 __device__ __forceinline__ T feq_k(int k, T rho, T ux, T uy, T u2, T wk,
                                    T ics2, T ics4) {
   T cu = (T)CX[k] * ux + (T)CY[k] * uy;
   return (wk * rho) *
          (((T)1 + ics2 * cu + ((T)0.5 * ics4) * cu * cu) - ((T)0.5 * ics2) * u2);
+}
+
+// Deviation form: feq_k - w_k = w (drho + rho (ics2 cu + 0.5 ics4 cu^2 - 0.5 ics2 u2))
+template <typename T>
+__device__ __forceinline__ T deq_k(int k, T drho, T rho, T ux, T uy, T u2, T wk,
+                                   T ics2, T ics4) {
+  T cu = (T)CX[k] * ux + (T)CY[k] * uy;
+  return wk * (drho + rho * (ics2 * cu + ((T)0.5 * ics4) * cu * cu - ((T)0.5 * ics2) * u2));
 }
 
 // Guo source without the (1 - omega/2) factor.
@@ -124,24 +222,31 @@ __device__ __forceinline__ T guo_k(int k, T ux, T uy, T fx, T fy, T wk,
   }
 }
 
-// Moments of the pre-collision populations g (fluid cell):
-// rho = sum g (from 0), F = rho * gravity or the per-cell field,
-// u = (sum c g) * inv + (0.5 F) * inv with inv = 1 / (rho + float_min).
-template <typename T, int FORCING>
+// Moments of the pre-collision values g (f, or d in deviation form):
+// rho = sum g (from 0) [+ 1 in deviation form], F = rho * gravity or the
+// per-cell field, u = (sum c g) * inv + (0.5 F) * inv, inv = 1/(rho + float_min).
+template <typename T, int FORCING, bool DEV>
 __device__ __forceinline__ void moments(const T (&g)[9], const d2q9_lattice& L,
-                                        T ffx, T ffy, T& rho, T& ux, T& uy,
-                                        T& fx, T& fy) {
+                                        const Consts<T>& K, T ffx, T ffy,
+                                        T& drho, T& rho, T& ux, T& uy, T& fx,
+                                        T& fy) {
   T r = (T)0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) r += g[k];
-  rho = r;
+  if constexpr (DEV) {
+    drho = r;
+    rho = (T)1 + r;
+  } else {
+    drho = (T)0;
+    rho = r;
+  }
   const bool field = L.force_source == D2Q9_FSRC_FIELD;
   if (field) {
     fx = ffx;
     fy = ffy;
   } else {
-    fx = rho * (T)L.gravity[0];
-    fy = rho * (T)L.gravity[1];
+    fx = rho * K.gx;
+    fy = rho * K.gy;
   }
   T sx = (T)0, sy = (T)0;
 #pragma unroll
@@ -149,7 +254,7 @@ __device__ __forceinline__ void moments(const T (&g)[9], const d2q9_lattice& L,
     sx += (T)CX[k] * g[k];
     sy += (T)CY[k] * g[k];
   }
-  T inv = (T)1 / (rho + (T)L.float_min);
+  T inv = (T)1 / (rho + K.fmin);
   if (FORCING != D2Q9_FORCING_NONE || field) {
     ux = sx * inv + ((T)0.5 * fx) * inv;
     uy = sy * inv + ((T)0.5 * fy) * inv;
@@ -161,18 +266,19 @@ __device__ __forceinline__ void moments(const T (&g)[9], const d2q9_lattice& L,
   }
 }
 
-// BGK collision in place: g <- (1 - om) g + om feq (+ (1 - om/2) Fbar).
-template <typename T, int FORCING>
-__device__ __forceinline__ void collide_bgk(T (&g)[9], const d2q9_lattice& L,
-                                            T rho, T ux, T uy, T fx, T fy) {
-  const T om = (T)L.omega;
-  const T ics2 = (T)L.inv_cs_2, ics4 = (T)L.inv_cs_4;
+// BGK collision in place: g <- (1 - om) g + om geq (+ (1 - om/2) Fbar).
+template <typename T, int FORCING, bool DEV>
+__device__ __forceinline__ void collide_bgk(T (&g)[9], const Consts<T>& K,
+                                            T drho, T rho, T ux, T uy, T fx, T fy) {
+  const T om = K.om;
+  const T ics2 = K.ics2, ics4 = K.ics4;
   const T u2 = ux * ux + uy * uy;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    T wk = (T)L.w[k];
-    T feq = feq_k<T>(k, rho, ux, uy, u2, wk, ics2, ics4);
-    T v = ((T)1 - om) * g[k] + om * feq;
+    T wk = K.w[k];
+    T eq = DEV ? deq_k<T>(k, drho, rho, ux, uy, u2, wk, ics2, ics4)
+               : feq_k<T>(k, rho, ux, uy, u2, wk, ics2, ics4);
+    T v = ((T)1 - om) * g[k] + om * eq;
     if constexpr (FORCING != D2Q9_FORCING_NONE) {
       v = v + ((T)1 - (T)0.5 * om) * guo_k<T, FORCING>(k, ux, uy, fx, fy, wk, ics2, ics4);
     }
@@ -180,17 +286,20 @@ __device__ __forceinline__ void collide_bgk(T (&g)[9], const d2q9_lattice& L,
   }
 }
 
-// MRT with Guo forcing in place (operation order of oracle mrt_collide).
-template <typename T, int FORCING>
-__device__ __forceinline__ void collide_mrt(T (&g)[9], const d2q9_lattice& L,
-                                            T rho, T ux, T uy, T fx, T fy) {
-  const T ics2 = (T)L.inv_cs_2, ics4 = (T)L.inv_cs_4;
+// MRT with Guo forcing in place (FP64 operation order of oracle
+// mrt_collide).  The moment map is linear and M w is a constant, so the
+// same code applies to deviations (m - meq is unchanged).
+template <typename T, int FORCING, bool DEV>
+__device__ __forceinline__ void collide_mrt(T (&g)[9], const Consts<T>& K,
+                                            T drho, T rho, T ux, T uy, T fx, T fy) {
+  const T ics2 = K.ics2, ics4 = K.ics4;
   const T u2 = ux * ux + uy * uy;
   T feq[9], fb[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    T wk = (T)L.w[k];
-    feq[k] = feq_k<T>(k, rho, ux, uy, u2, wk, ics2, ics4);
+    T wk = K.w[k];
+    feq[k] = DEV ? deq_k<T>(k, drho, rho, ux, uy, u2, wk, ics2, ics4)
+                 : feq_k<T>(k, rho, ux, uy, u2, wk, ics2, ics4);
     if constexpr (FORCING != D2Q9_FORCING_NONE)
       fb[k] = guo_k<T, FORCING>(k, ux, uy, fx, fy, wk, ics2, ics4);
     else
@@ -207,7 +316,7 @@ __device__ __forceinline__ void collide_mrt(T (&g)[9], const d2q9_lattice& L,
       b = b + (T)MM[i][k] * feq[k];
       if constexpr (FORCING != D2Q9_FORCING_NONE) d = d + (T)MM[i][k] * fb[k];
     }
-    T s = (T)L.rates[i];
+    T s = K.rates[i];
     T dm = s * (a - b) - ((T)1 - (T)0.5 * s) * d;
     gm[i] = dm * (T)(1.0 / (double)ROWNORM[i]);
   }
@@ -223,20 +332,25 @@ __device__ __forceinline__ void collide_mrt(T (&g)[9], const d2q9_lattice& L,
   }
 }
 
-template <typename T, int MODEL, int FORCING>
-__device__ __forceinline__ void collide(T (&g)[9], const d2q9_lattice& L, T rho,
-                                        T ux, T uy, T fx, T fy) {
+template <typename T, int MODEL, int FORCING, bool DEV>
+__device__ __forceinline__ void collide(T (&g)[9], const Consts<T>& K, T drho,
+                                        T rho, T ux, T uy, T fx, T fy) {
   if constexpr (MODEL == D2Q9_MRT)
-    collide_mrt<T, FORCING>(g, L, rho, ux, uy, fx, fy);
+    collide_mrt<T, FORCING, DEV>(g, K, drho, rho, ux, uy, fx, fy);
   else
-    collide_bgk<T, FORCING>(g, L, rho, ux, uy, fx, fy);
+    collide_bgk<T, FORCING, DEV>(g, K, drho, rho, ux, uy, fx, fy);
 }
 
 // ---------------------------------------------------------------------------
-// Streaming and boundary rules on the general (flagged) path.
+// Streaming and boundary rules on the general (flagged) path, one cell.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int64_t cell_index(const d2q9_lattice& L, int i, int j) {
-  return (int64_t)L.origin + (int64_t)i * L.pitch + j;
+template <int S>
+__device__ __forceinline__ typename Codec<S>::T ld(const d2q9_lattice& L,
+                                                   const Consts<typename Codec<S>::T>& K,
+                                                   const typename Codec<S>::raw* __restrict__ f,
+                                                   int i, int j, int k) {
+  constexpr int B = 64 * Codec<S>::CPL;
+  return Codec<S>::dec(f[pop_index<B>(L, i, j, k)], k, K);
 }
 
 // Value streamed into cell (i, j) in direction k (streaming_kernels.py:35-47
@@ -244,7 +358,8 @@ __device__ __forceinline__ int64_t cell_index(const d2q9_lattice& L, int i, int 
 // from the pull neighbour, or moving-wall bounce-back off a solid neighbour.
 template <int S>
 __device__ __forceinline__ typename Codec<S>::T pull(
-    const d2q9_lattice& L, const typename Codec<S>::raw* __restrict__ f,
+    const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K,
+    const typename Codec<S>::raw* __restrict__ f,
     const typename Codec<S>::T* __restrict__ mom, int i, int j, int64_t c,
     uint8_t fl, int k) {
   using T = typename Codec<S>::T;
@@ -255,9 +370,10 @@ __device__ __forceinline__ typename Codec<S>::T pull(
       const int ki = INV[k];
       T rho_c = mom[c];
       T usx = mom[L.plane + nb], usy = mom[2 * L.plane + nb];
-      T temp = ((((T)2 * (T)L.w[ki]) * rho_c) * (T)L.inv_cs_2) *
+      T temp = ((((T)2 * K.w[ki]) * rho_c) * K.ics2) *
                ((T)CX[ki] * usx + (T)CY[ki] * usy);
-      return Codec<S>::dec(f[(int64_t)ki * L.plane + c], ki, L) - temp;
+      // w[ki] == w[k]: the same expression holds for deviations
+      return ld<S>(L, K, f, i, j, ki) - temp;
     }
   }
   if (L.x_periodic) {
@@ -268,7 +384,7 @@ __device__ __forceinline__ typename Codec<S>::T pull(
     if (yn == 0) yn = L.ny;
     else if (yn == L.ny + 1) yn = 1;
   }
-  return Codec<S>::dec(f[(int64_t)k * L.plane + cell_index(L, xn, yn)], k, L);
+  return ld<S>(L, K, f, xn, yn, k);
 }
 
 // Overwrite the rule-bearing directions of boundary node (i, j)
@@ -276,12 +392,13 @@ __device__ __forceinline__ typename Codec<S>::T pull(
 // host, so the last element writing a direction is the one stored).
 template <int S>
 __device__ __forceinline__ void apply_bc(
-    const d2q9_lattice& L, const typename Codec<S>::raw* __restrict__ f,
+    const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K,
+    const typename Codec<S>::raw* __restrict__ f,
     const typename Codec<S>::T* __restrict__ mom, int i, int j, int64_t c,
     typename Codec<S>::T (&g)[9]) {
   using T = typename Codec<S>::T;
   const d2q9_bc_node node = L.bc_nodes[L.bc_slot[c]];
-  const T ics2 = (T)L.inv_cs_2, ics4 = (T)L.inv_cs_4;
+  const T ics2 = K.ics2, ics4 = K.ics4;
 #pragma unroll
   for (int d = 1; d < 9; ++d) {
     const int rule = node.rule[d];
@@ -290,12 +407,12 @@ __device__ __forceinline__ void apply_bc(
     const d2q9_bc_elem& e = L.bc_elems[rule >> 4];
     const int o = INV[d];
     if (kind == D2Q9_RULE_BB) {
-      g[d] = Codec<S>::dec(f[(int64_t)o * L.plane + c], o, L);
+      g[d] = ld<S>(L, K, f, i, j, o);
     } else if (kind == D2Q9_RULE_VEL) {
       T rho_c = mom[c];
-      T temp = ((((T)2 * (T)L.w[o]) * rho_c) * ics2) *
+      T temp = ((((T)2 * K.w[o]) * rho_c) * ics2) *
                ((T)CX[o] * (T)e.vec[0] + (T)CY[o] * (T)e.vec[1]);
-      g[d] = Codec<S>::dec(f[(int64_t)o * L.plane + c], o, L) - temp;
+      g[d] = ld<S>(L, K, f, i, j, o) - temp;
     } else if (kind == D2Q9_RULE_PRES) {
       int64_t nb = c + (int64_t)e.normal[0] * L.pitch + e.normal[1];
       T vx = mom[L.plane + c], vy = mom[2 * L.plane + c];
@@ -303,14 +420,23 @@ __device__ __forceinline__ void apply_bc(
       T ey = vy + (T)0.5 * (vy - mom[2 * L.plane + nb]);
       T u2 = ex * ex + ey * ey;
       T cu = (T)CX[o] * ex + (T)CY[o] * ey;
-      T temp = (((T)2 * (T)L.w[o]) * (T)e.scalar) *
-               (((T)1 + ((T)0.5 * ics4) * cu * cu) - ((T)0.5 * ics2) * u2);
-      g[d] = -Codec<S>::dec(f[(int64_t)o * L.plane + c], o, L) + temp;
+      const T rb = (T)e.scalar;
+      if constexpr (Codec<S>::dev) {
+        // d = -d*[o] + (2 w rho_b (1 + ...) - 2 w)
+        T temp = ((T)2 * K.w[o]) *
+                 ((rb - (T)1) + rb * (((T)0.5 * ics4) * cu * cu - ((T)0.5 * ics2) * u2));
+        g[d] = -ld<S>(L, K, f, i, j, o) + temp;
+      } else {
+        T temp = (((T)2 * K.w[o]) * rb) *
+                 (((T)1 + ((T)0.5 * ics4) * cu * cu) - ((T)0.5 * ics2) * u2);
+        g[d] = -ld<S>(L, K, f, i, j, o) + temp;
+      }
     } else {  // D2Q9_RULE_ZG
       int ib = i + e.normal[0], jb = j + e.normal[1];
       int64_t nb = cell_index(L, ib, jb);
       uint8_t fnb = L.flags[nb];
-      g[d] = (fnb & D2Q9_CELL_FLUID) ? pull<S>(L, f, mom, ib, jb, nb, fnb, d) : (T)0;
+      const T zero = Codec<S>::dev ? -K.w[d] : (T)0;
+      g[d] = (fnb & D2Q9_CELL_FLUID) ? pull<S>(L, K, f, mom, ib, jb, nb, fnb, d) : zero;
     }
   }
 }
@@ -321,6 +447,11 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
   const int per = n >> 3, rem = n & 7;
   const int x = b & 7, r = b >> 3;
   return x * per + (x < rem ? x : rem) + r;
+}
+
+template <typename V>
+__device__ __forceinline__ void store_nt(V v, V* p) {
+  __builtin_nontemporal_store(v, p);
 }
 
 }  // namespace d2q9

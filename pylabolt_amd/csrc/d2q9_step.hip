@@ -1,18 +1,22 @@
 // Single-lattice D2Q9 sweep kernels and their C ABI (include/d2q9.h).
 //
-// One thread per interior cell; a 256-thread workgroup covers a tile of
-// tile_j cells along y (contiguous in memory) by tile_i rows along x.  Every
-// population plane is read at a shifted position (pull streaming) and
-// written unshifted, so a wave's 64 lanes read and write 64 consecutive
-// elements of each of the 9 planes: every HBM access is coalesced.
+// A wave owns one AoSoA block (B = 64 * CPL cells along y) of one row; a
+// workgroup stacks tile_i such rows.  Each lane moves 8 bytes per direction
+// (1 double, 2 floats or 4 halves), reads the nine directions at their pull
+// offsets (the y-shift crosses lanes by a shuffle when CPL > 1) and writes
+// them back unshifted with non-temporal stores, so every HBM access of a
+// wave is one contiguous segment and the nine directions of a block sit
+// side by side.
 //
 // Tiles whose cells are all plain fluid (no solid neighbour, no boundary
-// rule, no periodic edge, no moment the next step reads) take a branch-free
-// path that never touches the flag array; the others read the per-cell flag
-// byte and apply bounce-back, periodic wrap and boundary rules in the same
-// sweep.
+// rule, no periodic edge, no moment the next step reads, fully inside the
+// lattice) take a branch-free path that never touches the flag array; the
+// others handle each cell with its flag byte: bounce-back, periodic wrap and
+// boundary rules in the same sweep.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -38,112 +42,355 @@ int check_hip(hipError_t e, const char* where) {
 
 enum { PHASE_COLLIDE = 0, PHASE_FUSED = 1, PHASE_STREAM = 2 };
 
-template <int S, int MODEL, int FORCING, int PHASE>
-__global__ __launch_bounds__(256) void k_sweep(
-    const d2q9_lattice L, const typename Codec<S>::raw* __restrict__ fsrc,
-    typename Codec<S>::raw* __restrict__ fdst,
-    const typename Codec<S>::T* __restrict__ msrc,
-    typename Codec<S>::T* __restrict__ mdst, int write_mom, int ntj,
-    int ntiles) {
-  using T = typename Codec<S>::T;
-  const int t = xcd_remap(blockIdx.x, ntiles);
-  const int tj = t % ntj, ti = t / ntj;
-  const int j = 1 + tj * L.tile_j + (int)threadIdx.x;
-  const int i = 1 + ti * L.tile_i + (int)threadIdx.y;
-  if (i > L.nx || j > L.ny) return;
-  const int64_t P = L.plane;
-  const int64_t c = cell_index(L, i, j);
-  const bool plain = L.tile_class != nullptr && L.tile_class[t] == 0;
+// 8-byte vector of CPL raw elements (one lane's share of one direction).
+template <typename R, int CPL>
+struct alignas(8) Vec8 {
+  R v[CPL];
+};
 
-  T g[9];
-  uint8_t fl = D2Q9_CELL_FLUID;
-  if (plain) {
+template <typename R, int CPL>
+__device__ __forceinline__ Vec8<R, CPL> load8(const R* p) {
+  static_assert(sizeof(R) * CPL == 8, "one lane moves 8 bytes");
+  uint64_t w = *reinterpret_cast<const uint64_t*>(p);
+  Vec8<R, CPL> r;
+  __builtin_memcpy(&r, &w, 8);
+  return r;
+}
+
+template <typename R, int CPL>
+__device__ __forceinline__ void store8_nt(R* p, const Vec8<R, CPL>& v) {
+  uint64_t w;
+  __builtin_memcpy(&w, &v, 8);
+  __builtin_nontemporal_store(w, reinterpret_cast<uint64_t*>(p));
+}
+
+// Pre-collision values of one flagged cell: bounce-back off solid
+// neighbours, periodic wrap and boundary rules (streaming_kernels.py:29-47,
+// MPI_kernels.py:34-99, fluid_boundary_kernels.py:5-152).
+template <int S, int PHASE>
+__device__ __forceinline__ void gather_cell(
+    const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K,
+    const typename Codec<S>::raw* __restrict__ fsrc,
+    const typename Codec<S>::T* __restrict__ msrc, int i, int j, int64_t c,
+    uint8_t fl, typename Codec<S>::T (&g)[9]) {
+  if (PHASE == PHASE_COLLIDE) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g[k] = ld<S>(L, K, fsrc, i, j, k);
+  } else {
+    g[0] = ld<S>(L, K, fsrc, i, j, 0);
+#pragma unroll
+    for (int k = 1; k < 9; ++k) g[k] = pull<S>(L, K, fsrc, msrc, i, j, c, fl, k);
+    if (fl & D2Q9_CELL_BC) apply_bc<S>(L, K, fsrc, msrc, i, j, c, g);
+  }
+}
+
+// Coalesced gather of one lane's CPL cells (j0 .. j0+CPL-1) of row i: the
+// nine directions of block tj+1 of the pull rows.  The y-shift is a shifted
+// address when CPL == 1; for CPL > 1 it is a cross-lane shuffle, and the two
+// cells at the ends of the wave's block take their neighbour from one
+// six-lane "halo" load (lanes 0..5 fetch, for directions 2,5,6 the element
+// before the block and for 4,7,8 the element after it) broadcast with
+// readlane.  All ten loads are issued before any is consumed.  Exact for
+// interior cells away from periodic edges; no address leaves the padded row.
+__device__ __forceinline__ uint32_t bits32(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ float from_bits32(uint32_t v) { return __uint_as_float(v); }
+
+template <typename T>
+__device__ __forceinline__ T wave_readlane(T v, int lane);
+template <>
+__device__ __forceinline__ float wave_readlane<float>(float v, int lane) {
+  return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v), lane));
+}
+
+template <int S, int PHASE>
+__device__ __forceinline__ void gather_block(
+    const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K,
+    const typename Codec<S>::raw* __restrict__ fsrc, int i, int tj, int lane,
+    int j0, typename Codec<S>::T (&g)[Codec<S>::CPL][9]) {
+  using C = Codec<S>;
+  using T = typename C::T;
+  using R = typename C::raw;
+  constexpr int CPL = C::CPL;
+  constexpr int B = 64 * CPL;
+  if constexpr (CPL == 1) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      int64_t src = c;
-      if (PHASE != PHASE_COLLIDE) src = c - (int64_t)CX[k] * L.pitch - CY[k];
-      g[k] = Codec<S>::dec(fsrc[(int64_t)k * P + src], k, L);
+      const int row = (PHASE == PHASE_COLLIDE) ? i : i - CX[k];
+      const int cy = (PHASE == PHASE_COLLIDE) ? 0 : CY[k];
+      const int jc = (j0 <= L.ny) ? j0 - cy : j0;
+      g[0][k] = C::dec(fsrc[pop_index<B>(L, row, jc, k)], k, K);
     }
   } else {
-    fl = L.flags[c];
-    if (!(fl & D2Q9_CELL_FLUID)) return;
-    if (PHASE == PHASE_COLLIDE) {
+    Vec8<R, CPL> v[9];
 #pragma unroll
-      for (int k = 0; k < 9; ++k) g[k] = Codec<S>::dec(fsrc[(int64_t)k * P + c], k, L);
-    } else {
-      g[0] = Codec<S>::dec(fsrc[c], 0, L);
+    for (int k = 0; k < 9; ++k) {
+      const int row = (PHASE == PHASE_COLLIDE) ? i : i - CX[k];
+      v[k] = load8<R, CPL>(fsrc + (((int64_t)row * L.nbj + tj + 1) * 9 + k) * B + lane * CPL);
+    }
+    R halo = R();
+    if (PHASE != PHASE_COLLIDE) {
+      // lane h < 6 fetches the end element of direction HK[h]
+      const int h = lane < 6 ? lane : 0;
+      const int kh = h == 0 ? 2 : h == 1 ? 5 : h == 2 ? 6 : h == 3 ? 4 : h == 4 ? 7 : 8;
+      const int cxh = h == 0 ? 0 : h == 1 ? 1 : h == 2 ? -1 : h == 3 ? 0 : h == 4 ? -1 : 1;
+      const int jf = 1 + tj * B;             // first cell of the block
+      int jh = h < 3 ? jf - 1 : jf + B;      // element before / after
+      if (jh > L.ny + 1) jh = jf;            // past the padded row: unused
+      halo = fsrc[pop_index<B>(L, i - cxh, jh, kh)];
+    }
 #pragma unroll
-      for (int k = 1; k < 9; ++k) g[k] = pull<S>(L, fsrc, msrc, i, j, c, fl, k);
-      if (fl & D2Q9_CELL_BC) apply_bc<S>(L, fsrc, msrc, i, j, c, g);
+    for (int k = 0; k < 9; ++k) {
+      const int cy = (PHASE == PHASE_COLLIDE) ? 0 : CY[k];
+      T x[CPL];
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) x[q] = C::dec(v[k].v[q], k, K);
+      if (cy == 0) {
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) g[q][k] = x[q];
+      } else {
+        const int h = k == 2 ? 0 : k == 5 ? 1 : k == 6 ? 2 : k == 4 ? 3 : k == 7 ? 4 : 5;
+        const T hv = wave_readlane<T>(C::dec(halo, k, K), h);
+        if (cy == 1) {  // cell j takes j - 1
+          T prev = __shfl_up(x[CPL - 1], 1, 64);
+          g[0][k] = lane == 0 ? hv : prev;
+#pragma unroll
+          for (int q = 1; q < CPL; ++q) g[q][k] = x[q - 1];
+        } else {  // cell j takes j + 1
+          T next = __shfl_down(x[0], 1, 64);
+#pragma unroll
+          for (int q = 0; q < CPL - 1; ++q) g[q][k] = x[q + 1];
+          g[CPL - 1][k] = lane == 63 ? hv : next;
+        }
+      }
     }
   }
+}
 
+// Moments + collision (or nothing for PHASE_STREAM) of one cell, encoded
+// into out[.][q]; rho/u stored when asked.
+template <int S, int MODEL, int FORCING, int PHASE>
+__device__ __forceinline__ void finish_cell(
+    const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K,
+    typename Codec<S>::T (&g)[9], typename Codec<S>::raw (&out)[9][Codec<S>::CPL],
+    int q, int64_t c, bool store_mom, typename Codec<S>::T* __restrict__ mdst) {
+  using C = Codec<S>;
+  using T = typename C::T;
   if constexpr (PHASE == PHASE_STREAM) {
 #pragma unroll
-    for (int k = 0; k < 9; ++k) fdst[(int64_t)k * P + c] = Codec<S>::enc(g[k], k, L);
-    return;
+    for (int k = 0; k < 9; ++k) out[k][q] = C::enc(g[k], k, K);
   } else {
     T ffx = (T)0, ffy = (T)0;
     if (L.force_source == D2Q9_FSRC_FIELD) {
       const T* ff = static_cast<const T*>(L.force_field);
       ffx = ff[c];
-      ffy = ff[P + c];
+      ffy = ff[L.plane + c];
     }
-    T rho, ux, uy, fx, fy;
-    moments<T, FORCING>(g, L, ffx, ffy, rho, ux, uy, fx, fy);
-    collide<T, MODEL, FORCING>(g, L, rho, ux, uy, fx, fy);
+    T drho, rho, ux, uy, fx, fy;
+    moments<T, FORCING, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
+    collide<T, MODEL, FORCING, C::dev>(g, K, drho, rho, ux, uy, fx, fy);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) fdst[(int64_t)k * P + c] = Codec<S>::enc(g[k], k, L);
-    if (write_mom || (fl & D2Q9_CELL_STORE_MOM)) {
+    for (int k = 0; k < 9; ++k) out[k][q] = C::enc(g[k], k, K);
+    if (store_mom) {
       mdst[c] = rho;
-      mdst[P + c] = ux;
-      mdst[2 * P + c] = uy;
+      mdst[L.plane + c] = ux;
+      mdst[2 * L.plane + c] = uy;
     }
   }
 }
 
-// f = feq(rho0, u0) on fluid cells, encoded zero on every other padded cell.
+template <int S>
+__device__ __forceinline__ void store_block(
+    const d2q9_lattice& L, typename Codec<S>::raw* __restrict__ fdst, int i,
+    int tj, int lane, const typename Codec<S>::raw (&out)[9][Codec<S>::CPL]) {
+  using R = typename Codec<S>::raw;
+  constexpr int CPL = Codec<S>::CPL;
+  constexpr int B = 64 * CPL;
+  const int64_t obase = (((int64_t)i * L.nbj + tj + 1) * 9) * B + lane * CPL;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    if constexpr (CPL == 1) {
+      store_nt(out[k][0], fdst + obase + (int64_t)k * B);
+    } else {
+      Vec8<R, CPL> v;
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) v.v[q] = out[k][q];
+      store8_nt<R, CPL>(fdst + obase + (int64_t)k * B, v);
+    }
+  }
+}
+
+// Bulk tiles (host-classified): every cell fluid and inside the lattice.
+// Every cell is swept as plain interior fluid and no flag is read; the
+// cells marked in bulk_masks (solid neighbour, boundary rule, stored moment,
+// periodic edge) are not stored here -- the fix-up pass writes them.
+template <int S, int MODEL, int FORCING, int PHASE>
+__global__ __launch_bounds__(256) void k_bulk(
+    const d2q9_lattice L, const typename Codec<S>::raw* __restrict__ fsrc,
+    typename Codec<S>::raw* __restrict__ fdst,
+    typename Codec<S>::T* __restrict__ mdst, int write_mom, int ntj) {
+  using C = Codec<S>;
+  using T = typename C::T;
+  constexpr int CPL = C::CPL;
+  constexpr int B = 64 * CPL;
+  const int p = xcd_remap(blockIdx.x, L.n_bulk);
+  const int t = L.tiles[p];
+  const int tj = t % ntj, ti = t / ntj;
+  const int lane = (int)threadIdx.x;
+  const int i = 1 + ti * (int)blockDim.y + (int)threadIdx.y;
+  const int j0 = 1 + tj * B + lane * CPL;
+  if (i > L.nx) return;  // whole wave (never for a bulk tile)
+  const Consts<T> K = load_consts<T>(L);
+  // special bits of this lane's cells
+  const uint64_t* mrow = L.bulk_masks + ((int64_t)p * blockDim.y + threadIdx.y) * CPL;
+  uint32_t skip = 0;
+  bool any = false;
+#pragma unroll
+  for (int w = 0; w < CPL; ++w) any |= mrow[w] != 0;
+  if (any) {
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      const int o = lane * CPL + q;
+      skip |= (uint32_t)((mrow[o >> 6] >> (o & 63)) & 1ull) << q;
+    }
+  }
+  T g[CPL][9];
+  gather_block<S, PHASE>(L, K, fsrc, i, tj, lane, j0, g);
+  typename C::raw out[9][CPL];
+#pragma unroll
+  for (int q = 0; q < CPL; ++q)
+    finish_cell<S, MODEL, FORCING, PHASE>(L, K, g[q], out, q, cell_index(L, i, j0 + q),
+                                          write_mom != 0 && !((skip >> q) & 1), mdst);
+  if (skip == 0) {
+    store_block<S>(L, fdst, i, tj, lane, out);
+  } else {
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      if ((skip >> q) & 1) continue;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) fdst[pop_index<B>(L, i, j0 + q, k)] = out[k][q];
+    }
+  }
+}
+
+// Second pass.  Workgroups [0, ngen) sweep the general tiles (those holding
+// solid cells or reaching past the lattice): coalesced gather for every
+// lane, the special cells re-gathered one by one, non-fluid cells written
+// back as the encoded zero.  Workgroups after that recompute the fix-up
+// cells of bulk tiles, one thread per cell, over what k_bulk stored.
+template <int S, int MODEL, int FORCING, int PHASE>
+__global__ __launch_bounds__(256) void k_general_fix(
+    const d2q9_lattice L, const typename Codec<S>::raw* __restrict__ fsrc,
+    typename Codec<S>::raw* __restrict__ fdst,
+    const typename Codec<S>::T* __restrict__ msrc,
+    typename Codec<S>::T* __restrict__ mdst, int write_mom, int ntj, int ngen,
+    int first) {
+  using C = Codec<S>;
+  using T = typename C::T;
+  constexpr int CPL = C::CPL;
+  constexpr int B = 64 * CPL;
+  const Consts<T> K = load_consts<T>(L);
+  if ((int)blockIdx.x >= ngen) {
+    const int n = ((int)blockIdx.x - ngen) * 256 + (int)(threadIdx.y * 64 + threadIdx.x);
+    if (n >= L.n_fix) return;
+    const int64_t c = L.fix_cells[n];
+    const int i = (int)((c - L.origin) / L.pitch), j = (int)((c - L.origin) % L.pitch);
+    const uint8_t fl = L.flags[c];
+    T g[9];
+    gather_cell<S, PHASE>(L, K, fsrc, msrc, i, j, c, fl, g);
+    typename C::raw out[9][CPL];
+    finish_cell<S, MODEL, FORCING, PHASE>(
+        L, K, g, out, 0, c, write_mom || (fl & D2Q9_CELL_STORE_MOM), mdst);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) fdst[pop_index<B>(L, i, j, k)] = out[k][0];
+    return;
+  }
+  const int r = xcd_remap(blockIdx.x, ngen);
+  const int t = L.tiles ? L.tiles[first + r] : r;
+  const int tj = t % ntj, ti = t / ntj;
+  const int lane = (int)threadIdx.x;
+  const int i = 1 + ti * (int)blockDim.y + (int)threadIdx.y;
+  const int j0 = 1 + tj * B + lane * CPL;
+  if (i > L.nx) return;  // whole wave
+  T g[CPL][9];
+  gather_block<S, PHASE>(L, K, fsrc, i, tj, lane, j0, g);
+  uint8_t fl[CPL];
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const int j = j0 + q;
+    fl[q] = 0;
+    if (j > L.ny) continue;
+    const int64_t c = cell_index(L, i, j);
+    fl[q] = L.flags[c];
+    if (!(fl[q] & D2Q9_CELL_FLUID)) continue;
+    const bool edge = (L.x_periodic && (i == 1 || i == L.nx)) ||
+                      (L.y_periodic && (j == 1 || j == L.ny));
+    if (fl[q] != D2Q9_CELL_FLUID || (edge && PHASE != PHASE_COLLIDE))
+      gather_cell<S, PHASE>(L, K, fsrc, msrc, i, j, c, fl[q], g[q]);
+  }
+  typename C::raw out[9][CPL];
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    if (!(fl[q] & D2Q9_CELL_FLUID)) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) out[k][q] = enc_zero<S>(k, K);
+      continue;
+    }
+    finish_cell<S, MODEL, FORCING, PHASE>(
+        L, K, g[q], out, q, cell_index(L, i, j0 + q),
+        write_mom || (fl[q] & D2Q9_CELL_STORE_MOM), mdst);
+  }
+  // Cells beyond ny in a partial block: the ghost column ny+1 gets its
+  // encoded zero back, the block padding after it is never read.
+  if (CPL == 1 && j0 > L.ny + 1) return;
+  store_block<S>(L, fdst, i, tj, lane, out);
+}
+
+// f = feq(rho0, u0) on fluid cells, the encoded zero on every other padded
+// cell, in both buffers.
 template <int S>
 __global__ __launch_bounds__(256) void k_init(const d2q9_lattice L,
                                               typename Codec<S>::raw* fa,
                                               typename Codec<S>::raw* fb,
                                               const typename Codec<S>::T* mom) {
-  using T = typename Codec<S>::T;
-  const int64_t n = (int64_t)(L.nx + 2) * L.pitch;
+  using C = Codec<S>;
+  using T = typename C::T;
+  constexpr int B = 64 * C::CPL;
+  const int64_t n = (int64_t)(L.nx + 2) * (L.ny + 2);
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  const int i = (int)(e / L.pitch), j = (int)(e % L.pitch);
-  if (j > L.ny + 1) return;
+  const int i = (int)(e / (L.ny + 2)), j = (int)(e % (L.ny + 2));
   const int64_t c = cell_index(L, i, j);
   const int64_t P = L.plane;
+  const Consts<T> K = load_consts<T>(L);
   if (L.flags[c] & D2Q9_CELL_FLUID) {
     const T rho = mom[c], ux = mom[P + c], uy = mom[2 * P + c];
-    const T ics2 = (T)L.inv_cs_2, ics4 = (T)L.inv_cs_4;
+    const T ics2 = K.ics2, ics4 = K.ics4;
     const T u2 = ux * ux + uy * uy;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      auto r = Codec<S>::enc(feq_k<T>(k, rho, ux, uy, u2, (T)L.w[k], ics2, ics4), k, L);
-      fa[k * P + c] = r;
-      fb[k * P + c] = r;
+      T v = C::dev ? deq_k<T>(k, rho - (T)1, rho, ux, uy, u2, K.w[k], ics2, ics4)
+                   : feq_k<T>(k, rho, ux, uy, u2, K.w[k], ics2, ics4);
+      auto r = C::enc(v, k, K);
+      fa[pop_index<B>(L, i, j, k)] = r;
+      fb[pop_index<B>(L, i, j, k)] = r;
     }
   } else {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      auto z = Codec<S>::zero(k, L);
-      fa[k * P + c] = z;
-      fb[k * P + c] = z;
+      auto z = enc_zero<S>(k, K);
+      fa[pop_index<B>(L, i, j, k)] = z;
+      fb[pop_index<B>(L, i, j, k)] = z;
     }
   }
 }
 
-// rho/u of the stored populations on every fluid cell (no force term: these
-// are populations after streaming; the force of the next step is added by
-// the sweep itself).
+// rho/u of the stored populations on every fluid cell.
 template <int S>
 __global__ __launch_bounds__(256) void k_moments(const d2q9_lattice L,
                                                  const typename Codec<S>::raw* f,
                                                  typename Codec<S>::T* mom) {
-  using T = typename Codec<S>::T;
+  using C = Codec<S>;
+  using T = typename C::T;
   const int64_t n = (int64_t)L.nx * L.ny;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
@@ -151,20 +398,21 @@ __global__ __launch_bounds__(256) void k_moments(const d2q9_lattice L,
   const int64_t c = cell_index(L, i, j);
   if (!(L.flags[c] & D2Q9_CELL_FLUID)) return;
   const int64_t P = L.plane;
+  const Consts<T> K = load_consts<T>(L);
   T g[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) g[k] = Codec<S>::dec(f[k * P + c], k, L);
+  for (int k = 0; k < 9; ++k) g[k] = ld<S>(L, K, f, i, j, k);
   T ffx = (T)0, ffy = (T)0;
   if (L.force_source == D2Q9_FSRC_FIELD) {
     const T* ff = static_cast<const T*>(L.force_field);
     ffx = ff[c];
     ffy = ff[P + c];
   }
-  T rho, ux, uy, fx, fy;
+  T drho, rho, ux, uy, fx, fy;
   if (L.forcing == D2Q9_FORCING_NONE)
-    moments<T, D2Q9_FORCING_NONE>(g, L, ffx, ffy, rho, ux, uy, fx, fy);
+    moments<T, D2Q9_FORCING_NONE, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
   else
-    moments<T, D2Q9_GUO_LINEAR>(g, L, ffx, ffy, rho, ux, uy, fx, fy);
+    moments<T, D2Q9_GUO_LINEAR, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
   mom[c] = rho;
   mom[P + c] = ux;
   mom[2 * P + c] = uy;
@@ -201,7 +449,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_mass_partial(
     if (!(L.flags[c] & D2Q9_CELL_FLUID)) continue;
     double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) s += Codec<S>::dec64(f[k * L.plane + c], k, L);
+    for (int k = 0; k < 9; ++k)
+      s += Codec<S>::f64(f[pop_index<64 * Codec<S>::CPL>(L, i, j, k)], k, L);
     acc += s;
   }
   double r = block_sum(acc, sh);
@@ -249,9 +498,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_residue_partial(
 
 __global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src,
                                                 uint4* __restrict__ dst, int64_t n) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * blockDim.x)
-    dst[e] = src[e];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const uint4 v = src[e];
+  __builtin_nontemporal_store(v.x, &dst[e].x);
+  __builtin_nontemporal_store(v.y, &dst[e].y);
+  __builtin_nontemporal_store(v.z, &dst[e].z);
+  __builtin_nontemporal_store(v.w, &dst[e].w);
 }
 
 int validate(const d2q9_lattice* L) {
@@ -260,8 +513,17 @@ int validate(const d2q9_lattice* L) {
   if (L->pitch < L->ny + 2) return fail(D2Q9_E_ARG, "pitch < ny + 2");
   if (L->origin < 0 || L->plane < (int64_t)L->origin + (int64_t)(L->nx + 2) * L->pitch)
     return fail(D2Q9_E_ARG, "plane too small for origin + (nx + 2) * pitch");
-  if (L->tile_j < 1 || L->tile_i < 1 || L->tile_j * L->tile_i != 256)
-    return fail(D2Q9_E_ARG, "tile_j * tile_i must be 256");
+  if (L->tile_i < 1 || L->tile_i > 4) return fail(D2Q9_E_ARG, "tile_i must be 1..4");
+  if (L->storage < 0 || L->storage > 3) return fail(D2Q9_E_ARG, "bad storage");
+  {
+    const int cpl = L->storage == D2Q9_STORE_F64 ? 1 : (L->storage == D2Q9_STORE_F32 ? 2 : 4);
+    const int B = 64 * cpl;
+    if (L->cpl != cpl) return fail(D2Q9_E_ARG, "cpl does not match the storage");
+    if (L->tile_j != B) return fail(D2Q9_E_ARG, "tile_j must be 64 * cpl");
+    if (L->nbj < (L->ny + B) / B + 1) return fail(D2Q9_E_ARG, "nbj too small");
+    if (L->pop_len < (int64_t)(L->nx + 2) * L->nbj * 9 * B)
+      return fail(D2Q9_E_ARG, "pop_len too small");
+  }
   if (L->storage < 0 || L->storage > 3) return fail(D2Q9_E_ARG, "bad storage");
   if (L->collision < 0 || L->collision > 1) return fail(D2Q9_E_ARG, "bad collision");
   if (L->forcing < 0 || L->forcing > 2) return fail(D2Q9_E_ARG, "bad forcing");
@@ -273,6 +535,37 @@ int validate(const d2q9_lattice* L) {
   return 0;
 }
 
+// One auxiliary stream and two events per device, created on first use and
+// kept for the life of the process (default priority).
+struct Aux {
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+constexpr int MAX_DEVICES = 64;
+Aux g_aux[MAX_DEVICES];
+std::mutex g_aux_mutex;
+
+int aux_for_current_device(Aux** out) {
+  int dev = 0;
+  if (int r = check_hip(hipGetDevice(&dev), "hipGetDevice")) return r;
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(D2Q9_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_aux_mutex);
+  Aux& a = g_aux[dev];
+  if (!a.stream) {
+    if (int r = check_hip(hipStreamCreateWithFlags(&a.stream, hipStreamNonBlocking),
+                          "hipStreamCreateWithFlags"))
+      return r;
+    if (int r = check_hip(hipEventCreateWithFlags(&a.fork, hipEventDisableTiming),
+                          "hipEventCreateWithFlags"))
+      return r;
+    if (int r = check_hip(hipEventCreateWithFlags(&a.join, hipEventDisableTiming),
+                          "hipEventCreateWithFlags"))
+      return r;
+  }
+  *out = &a;
+  return 0;
+}
+
 template <int S, int MODEL, int FORCING, int PHASE>
 int launch_sweep(const d2q9_lattice* L, const void* fsrc, void* fdst,
                  const void* msrc, void* mdst, int write_mom, hipStream_t s) {
@@ -281,13 +574,49 @@ int launch_sweep(const d2q9_lattice* L, const void* fsrc, void* fdst,
   const int nti = (L->nx + L->tile_i - 1) / L->tile_i;
   const int64_t ntiles = (int64_t)ntj * nti;
   if (ntiles > 0x7fffffff) return fail(D2Q9_E_ARG, "lattice too large");
-  dim3 block(L->tile_j, L->tile_i);
-  hipLaunchKernelGGL((k_sweep<S, MODEL, FORCING, PHASE>), dim3((unsigned)ntiles),
-                     block, 0, s, *L, static_cast<const typename C::raw*>(fsrc),
-                     static_cast<typename C::raw*>(fdst),
-                     static_cast<const typename C::T*>(msrc),
-                     static_cast<typename C::T*>(mdst), write_mom, ntj, (int)ntiles);
-  return check_hip(hipGetLastError(), "k_sweep launch");
+  dim3 block(64, L->tile_i);
+  auto fs = static_cast<const typename C::raw*>(fsrc);
+  auto fd = static_cast<typename C::raw*>(fdst);
+  auto ms = static_cast<const typename C::T*>(msrc);
+  auto md = static_cast<typename C::T*>(mdst);
+  if (L->tiles == nullptr) {
+    hipLaunchKernelGGL((k_general_fix<S, MODEL, FORCING, PHASE>), dim3((unsigned)ntiles),
+                       block, 0, s, *L, fs, fd, ms, md, write_mom, ntj, (int)ntiles, 0);
+    return check_hip(hipGetLastError(), "k_general_fix launch");
+  }
+  if ((int64_t)L->n_bulk + L->n_general != ntiles)
+    return fail(D2Q9_E_ARG, "n_bulk + n_general must equal the tile count");
+  if (L->n_fix > 0 && !L->fix_cells) return fail(D2Q9_E_ARG, "null fix_cells");
+  if (L->n_bulk > 0 && !L->bulk_masks) return fail(D2Q9_E_ARG, "null bulk_masks");
+  const int nfixb = (L->n_fix + 255) / 256;
+  if (L->tile_i != 4 && nfixb > 0)
+    return fail(D2Q9_E_ARG, "fix-up cells need 256-thread workgroups (tile_i 4)");
+  hipStream_t aux = s;
+  Aux* a = nullptr;
+  if (L->n_bulk > 0 && L->n_general + nfixb > 0) {
+    if (int r = aux_for_current_device(&a)) return r;
+    aux = a->stream;
+    if (int r = check_hip(hipEventRecord(a->fork, s), "hipEventRecord fork")) return r;
+    if (int r = check_hip(hipStreamWaitEvent(aux, a->fork, 0), "hipStreamWaitEvent fork"))
+      return r;
+  }
+  if (L->n_general + nfixb > 0) {
+    hipLaunchKernelGGL((k_general_fix<S, MODEL, FORCING, PHASE>),
+                       dim3((unsigned)(L->n_general + nfixb)), block, 0, aux, *L, fs, fd, ms,
+                       md, write_mom, ntj, L->n_general, L->n_bulk);
+    if (int r = check_hip(hipGetLastError(), "k_general_fix launch")) return r;
+  }
+  if (L->n_bulk > 0) {
+    hipLaunchKernelGGL((k_bulk<S, MODEL, FORCING, PHASE>), dim3((unsigned)L->n_bulk), block,
+                       0, s, *L, fs, fd, md, write_mom, ntj);
+    if (int r = check_hip(hipGetLastError(), "k_bulk launch")) return r;
+  }
+  if (a) {
+    if (int r = check_hip(hipEventRecord(a->join, aux), "hipEventRecord join")) return r;
+    if (int r = check_hip(hipStreamWaitEvent(s, a->join, 0), "hipStreamWaitEvent join"))
+      return r;
+  }
+  return 0;
 }
 
 template <int S, int PHASE>
@@ -349,7 +678,7 @@ int d2q9_init_eq(const d2q9_lattice* L, void* f_a, void* f_b, const void* mom,
   if (int r = validate(L)) return r;
   if (!f_a || !f_b || !mom) return fail(D2Q9_E_ARG, "null buffer");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const int64_t n = (int64_t)(L->nx + 2) * L->pitch;
+  const int64_t n = (int64_t)(L->nx + 2) * (L->ny + 2);
   const unsigned blocks = (unsigned)((n + 255) / 256);
   switch (L->storage) {
 #define D2Q9_INIT(SV)                                                          \
@@ -460,8 +789,8 @@ int d2q9_copy(void* dst, const void* src, int64_t bytes, void* stream) {
     return fail(D2Q9_E_ARG, "copy needs non-null buffers and a multiple of 16 bytes");
   const int64_t n = bytes / 16;
   int64_t blocks = (n + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  if (blocks < 1) blocks = 1;
+  if (blocks > 0x7fffffff) return fail(D2Q9_E_TOO_LARGE, "copy too large");
+  if (blocks < 1) return 0;
   hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0,
                      static_cast<hipStream_t>(stream), static_cast<const uint4*>(src),
                      static_cast<uint4*>(dst), n);
@@ -502,5 +831,26 @@ int d2q9_graph_destroy(void* graph_exec) {
 }
 
 const char* d2q9_last_error(void) { return g_last_error.c_str(); }
+
+int d2q9_abi_layout(int64_t* out, int n) {
+  const int64_t v[] = {
+      (int64_t)sizeof(d2q9_lattice), (int64_t)sizeof(d2q9_bc_elem),
+      (int64_t)sizeof(d2q9_bc_node), (int64_t)offsetof(d2q9_lattice, plane),
+      (int64_t)offsetof(d2q9_lattice, cpl), (int64_t)offsetof(d2q9_lattice, pop_len),
+      (int64_t)offsetof(d2q9_lattice, storage), (int64_t)offsetof(d2q9_lattice, tile_j),
+      (int64_t)offsetof(d2q9_lattice, omega), (int64_t)offsetof(d2q9_lattice, w),
+      (int64_t)offsetof(d2q9_lattice, gravity), (int64_t)offsetof(d2q9_lattice, rates),
+      (int64_t)offsetof(d2q9_lattice, dev_scale), (int64_t)offsetof(d2q9_lattice, cf),
+      (int64_t)offsetof(d2q9_lattice, flags), (int64_t)offsetof(d2q9_lattice, tiles),
+      (int64_t)offsetof(d2q9_lattice, n_bulk), (int64_t)offsetof(d2q9_lattice, fix_cells),
+      (int64_t)offsetof(d2q9_lattice, n_fix), (int64_t)offsetof(d2q9_lattice, bulk_masks),
+      (int64_t)offsetof(d2q9_lattice, bc_slot),
+      (int64_t)offsetof(d2q9_lattice, bc_nodes), (int64_t)offsetof(d2q9_lattice, bc_elems),
+      (int64_t)offsetof(d2q9_lattice, force_field)};
+  const int m = (int)(sizeof(v) / sizeof(v[0]));
+  if (!out || n < m) return fail(D2Q9_E_ARG, "abi layout needs 24 slots");
+  for (int q = 0; q < m; ++q) out[q] = v[q];
+  return m;
+}
 
 }  // extern "C"

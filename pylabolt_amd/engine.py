@@ -55,6 +55,24 @@ def geometry(nx, ny):
     return pitch, plane
 
 
+CPL = {nt.STORE_F64: 1, nt.STORE_F32: 2, nt.STORE_F16DEV: 4,
+       nt.STORE_BF16DEV: 4}
+
+
+def pop_geometry(nx, ny, cpl):
+    """(B, nbj, pop_len) of the AoSoA population layout."""
+    B = 64 * cpl
+    nbj = (ny + B) // B + 1
+    return B, nbj, (nx + 2) * nbj * 9 * B
+
+
+def pop_aos(buf, nx, ny, B, nbj):
+    """AoSoA buffer -> reference AoS view [(nx+2)*(ny+2), 9] (a copy)."""
+    t = buf.reshape(nx + 2, nbj, 9, B).permute(0, 2, 1, 3)
+    t = t.reshape(nx + 2, 9, nbj * B)[:, :, B - 1:B - 1 + ny + 2]
+    return t.permute(0, 2, 1).reshape(-1, 9)
+
+
 def padded_view(flat, q, nx, ny, pitch, plane):
     """View of plane ``q`` of ``flat`` as the padded (nx+2, ny+2) grid."""
     base = q * plane + ORIGIN
@@ -130,26 +148,62 @@ class CaseTables:
         self.set_tile(tile_j, tile_i)
 
     def set_tile(self, tile_j, tile_i):
-        """Classify tiles of tile_j x tile_i interior cells: 0 = plain."""
-        if tile_j * tile_i != 256:
-            raise ValueError("tile_j * tile_i must be 256")
+        """Classify tiles of tile_j x tile_i interior cells.
+
+        Bulk tiles (class 0) hold only fluid cells inside the lattice; the
+        bulk sweep treats all their cells as plain interior fluid, and the
+        cells of bulk tiles that are not -- solid neighbour, boundary rule,
+        stored moment, periodic edge -- go to the fix-up list.  General
+        tiles (class 1) hold a solid cell or reach past the lattice.
+        """
+        if tile_j % 64 or not 1 <= tile_i <= 4:
+            raise ValueError("tile_j must be 64 * cpl and tile_i 1..4")
         self.tile_j, self.tile_i = int(tile_j), int(tile_i)
         nx, ny = self.nx, self.ny
         inner = self.flags_pad[1:-1, 1:-1]
-        plain = inner == nt.CELL_FLUID
+        fluid = (inner & nt.CELL_FLUID) != 0
+        special = inner != nt.CELL_FLUID
         if self.x_periodic:
-            plain[0, :] = False
-            plain[-1, :] = False
+            special[0, :] = True
+            special[-1, :] = True
         if self.y_periodic:
-            plain[:, 0] = False
-            plain[:, -1] = False
+            special[:, 0] = True
+            special[:, -1] = True
+        special &= fluid
         ntj = -(-ny // self.tile_j)
         nti = -(-nx // self.tile_i)
-        padded = np.ones((nti * self.tile_i, ntj * self.tile_j), dtype=bool)
-        padded[:nx, :ny] = plain
+        padded = np.zeros((nti * self.tile_i, ntj * self.tile_j), dtype=bool)
+        padded[:nx, :ny] = fluid
         tiles = padded.reshape(nti, self.tile_i, ntj, self.tile_j).all(axis=(1, 3))
         self.tile_class = np.where(tiles, 0, 1).astype(np.uint8).ravel()
         self.ntj, self.nti = ntj, nti
+        ids = np.arange(self.tile_class.size, dtype=np.int32)
+        bulk_ids = ids[self.tile_class == 0]
+        general_ids = ids[self.tile_class != 0]
+        self.tile_list = np.concatenate([bulk_ids, general_ids]).astype(np.int32)
+        self.n_bulk, self.n_general = int(bulk_ids.size), int(general_ids.size)
+        # fix-up cells: special cells lying in bulk tiles
+        in_bulk = np.repeat(np.repeat(tiles, self.tile_i, axis=0),
+                            self.tile_j, axis=1)[:nx, :ny]
+        # special-cell bits per (bulk tile, row, 64-cell word)
+        spad = np.zeros((nti * self.tile_i, ntj * self.tile_j), dtype=bool)
+        spad[:nx, :ny] = special
+        words = self.tile_j // 64
+        bits = spad.reshape(nti, self.tile_i, ntj, words, 64)
+        weights = (np.uint64(1) << np.arange(64, dtype=np.uint64))
+        m = (bits.astype(np.uint64) * weights).sum(axis=-1, dtype=np.uint64)
+        m = m.transpose(0, 2, 1, 3).reshape(nti * ntj, self.tile_i, words)
+        self.bulk_masks = np.ascontiguousarray(m[bulk_ids]).reshape(-1)
+        if self.bulk_masks.size == 0:
+            self.bulk_masks = np.zeros(1, dtype=np.uint64)
+        ii, jj = np.nonzero(special & in_bulk)
+        pitch = int(math.ceil((ny + 2) / ALIGN) * ALIGN)
+        self.fix_cells = (ORIGIN + (ii + 1) * pitch + (jj + 1)).astype(np.int32)
+        if self.fix_cells.size == 0:
+            self.fix_cells = np.zeros(1, dtype=np.int32)
+            self.n_fix = 0
+        else:
+            self.n_fix = int(self.fix_cells.size)
         return self.tile_class
 
 
@@ -182,7 +236,7 @@ class D2Q9Engine:
     def __init__(self, nx, ny, solid_pad, density, velocity, elements,
                  x_periodic, y_periodic, model, forcing, omega, gravity,
                  inv_cs_2, inv_cs_4, weights, float_min, storage="f64",
-                 force_field=None, tile=(256, 1), rates=None, dev_scale=None,
+                 force_field=None, tile=(64, 4), rates=None, dev_scale=None,
                  device=None, graph_steps=16):
         self.lib = nt.lib()
         self.device = (torch.device(device) if device is not None else
@@ -195,15 +249,22 @@ class D2Q9Engine:
         self.model = model
         self.forcing = forcing
         self.weights = np.asarray(weights, dtype=np.float64)
+        self.cpl = CPL[self.storage]
+        self.B, self.nbj, self.pop_len = pop_geometry(self.nx, self.ny,
+                                                      self.cpl)
+        self.tile_i = int(tile[1]) if tile is not None else 4
+        if self.tile_i != 4:
+            raise ValueError("the sweep runs 256-thread workgroups: 4 rows "
+                             "per tile")
         self.tables = CaseTables(self.nx, self.ny, solid_pad, elements,
-                                 x_periodic, y_periodic, tile[0], tile[1])
+                                 x_periodic, y_periodic, self.B, self.tile_i)
         self.stream = torch.cuda.Stream(device=self.device)
         self._stream_ptr = ctypes.c_void_p(self.stream.cuda_stream)
         dev = self.device
         P = self.plane
         with torch.cuda.stream(self.stream):
-            self.f = [torch.empty(9 * P, dtype=self.raw_dtype, device=dev)
-                      for _ in range(2)]
+            self.f = [torch.empty(self.pop_len, dtype=self.raw_dtype,
+                                  device=dev) for _ in range(2)]
             self.mom = [torch.zeros(3 * P, dtype=self.ctype, device=dev)
                         for _ in range(2)]
             self.flags = self._upload_plane(self.tables.flags_pad, torch.uint8)
@@ -220,7 +281,10 @@ class D2Q9Engine:
             raw = bytes(elems)
             self.bc_elems = torch.frombuffer(bytearray(raw),
                                              dtype=torch.uint8).to(dev)
-            self.tile_class = torch.from_numpy(self.tables.tile_class).to(dev)
+            self.tile_list = torch.from_numpy(self.tables.tile_list).to(dev)
+            self.fix_cells = torch.from_numpy(self.tables.fix_cells).to(dev)
+            self.bulk_masks = torch.from_numpy(
+                self.tables.bulk_masks.view(np.int64)).to(dev)
             self.force_field = None
             if force_field is not None:
                 self.force_field = self._pitched_vector(force_field)
@@ -229,6 +293,7 @@ class D2Q9Engine:
         L = nt.Lattice()
         L.nx, L.ny, L.pitch, L.origin, L.plane = (self.nx, self.ny, self.pitch,
                                                  ORIGIN, P)
+        L.cpl, L.nbj, L.pop_len = self.cpl, self.nbj, self.pop_len
         L.storage = self.storage
         L.collision = nt.MODEL_CODES[model]
         L.forcing = nt.FORCING_CODES[forcing]
@@ -246,11 +311,28 @@ class D2Q9Engine:
         for k in range(9):
             L.rates[k] = float(rates[k])
         if dev_scale is None:
-            dev_scale = [1.0] * 9
+            # FP16 keeps (f - w) / w * 1024 clear of its subnormal range
+            if self.storage == nt.STORE_F16DEV:
+                dev_scale = [1024.0 / float(w) for w in self.weights]
+            else:
+                dev_scale = [1.0] * 9
         for k in range(9):
             L.dev_scale[k] = float(dev_scale[k])
+        cf = L.cf
+        for k in range(9):
+            cf.w[k] = L.w[k]
+            cf.rates[k] = L.rates[k]
+            cf.scale[k] = L.dev_scale[k]
+            cf.inv_scale[k] = 1.0 / L.dev_scale[k]
+        cf.omega, cf.inv_cs_2, cf.inv_cs_4 = L.omega, L.inv_cs_2, L.inv_cs_4
+        cf.float_min = L.float_min
+        cf.gravity[0], cf.gravity[1] = L.gravity[0], L.gravity[1]
         L.flags = self.flags.data_ptr()
-        L.tile_class = self.tile_class.data_ptr()
+        L.tiles = self.tile_list.data_ptr()
+        L.n_bulk, L.n_general = self.tables.n_bulk, self.tables.n_general
+        L.fix_cells = self.fix_cells.data_ptr()
+        L.n_fix = self.tables.n_fix
+        L.bulk_masks = self.bulk_masks.data_ptr()
         if self.tables.n_bc_nodes:
             L.bc_slot = self.bc_slot.data_ptr()
             L.bc_nodes = self.bc_nodes.data_ptr()
@@ -449,14 +531,16 @@ class D2Q9Engine:
     def _decode_aos(self, idx):
         """Populations of buffer ``idx`` as a float64 numpy [size, 9]."""
         with torch.cuda.stream(self.stream):
-            planes = []
-            for k in range(9):
-                g = self._grid(self.f[idx], k).to(torch.float64)
-                if self.storage in (nt.STORE_F16DEV, nt.STORE_BF16DEV):
-                    g = self.weights[k] + g / float(self.L.dev_scale[k])
-                planes.append(g)
-            aos = torch.stack(planes, dim=-1).reshape(-1, 9).cpu().numpy()
-        if self.storage in (nt.STORE_F16DEV, nt.STORE_BF16DEV):
+            aos = pop_aos(self.f[idx], self.nx, self.ny, self.B,
+                          self.nbj).to(torch.float64)
+            if self.storage != nt.STORE_F64:
+                w = torch.tensor(self.weights, dtype=torch.float64,
+                                 device=aos.device)
+                sc = torch.tensor([self.L.dev_scale[k] for k in range(9)],
+                                  dtype=torch.float64, device=aos.device)
+                aos = w + aos / sc
+            aos = aos.cpu().numpy()
+        if self.storage != nt.STORE_F64:
             aos[~self.tables.fluid_pad.ravel()] = 0.0
         return aos
 

@@ -66,7 +66,7 @@ class Solver:
     """
 
     def __init__(self, comm=None, backend="gpu", n_threads=None,
-                 simulation=None, storage="f64", tile=(256, 1),
+                 simulation=None, storage="f64", tile=(64, 4),
                  graph_steps=16, verbose=True, output_root="."):
         comm = comm if comm is not None else SingleRankComm()
         mpi_rank = comm.Get_rank()

@@ -19,10 +19,10 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def _engine(sim, tile=(256, 1), graph_steps=16, storage="f64"):
+def _engine(sim, graph_steps=16, storage="f64"):
     from pylabolt_amd.solvers.fluidLB import Solver
-    s = Solver(simulation=sim, storage=storage, tile=tile,
-               graph_steps=graph_steps, verbose=False)
+    s = Solver(simulation=sim, storage=storage, graph_steps=graph_steps,
+               verbose=False)
     s.set_backend()
     return s
 
@@ -56,24 +56,29 @@ SMALL = {
     "channel_mrt_guo1": lambda: cases.poiseuille_channel(
         8, 6, forcing="guo_linear"),
     "cylinder_zg_pressure": lambda: cases.cylinder(40, 20, diameter=8, xc=12),
+    # wide enough (ny >= 64) for bulk tiles whose boundary, near-solid and
+    # stored-moment cells go through the fix-up pass
+    "cavity_wide": lambda: cases.lid_driven_cavity(9, 192),
+    "channel_wide_mrt": lambda: cases.poiseuille_channel(12, 128),
+    "cylinder_wide": lambda: cases.cylinder(44, 130, diameter=10, xc=14),
 }
 
 
 @pytest.mark.parametrize("name", sorted(SMALL))
-@pytest.mark.parametrize("tile", [(256, 1), (64, 4)])
-def test_small_cases_exact(name, tile):
+@pytest.mark.parametrize("graph_steps", [2, 4])
+def test_small_cases_exact(name, graph_steps):
     sim = SMALL[name]()
     if name.startswith("channel"):
         sim.forcing_dict["gravity"] = [2e-5, 3e-6]
     case = oracle_case(sim)
     O.initialize_pop(case)
-    solver = _engine(sim, tile=tile, graph_steps=4)
+    solver = _engine(sim, graph_steps=graph_steps)
     done = 0
     for chunk in (1, 1, 5, 13):
         solver.advance(chunk)
         O.run(case, chunk)
         done += chunk
-        _compare(solver, case, f"{name} tile={tile} step={done}")
+        _compare(solver, case, f"{name} graph={graph_steps} step={done}")
 
 
 def test_single_time_step_api():
@@ -133,7 +138,7 @@ def test_graph_replay_long_periodic_plain_tiles():
                                               0.02 * np.cos(i / 5.0))}
     case = oracle_case(sim)
     O.initialize_pop(case)
-    solver = _engine(sim, tile=(64, 4), graph_steps=8)
+    solver = _engine(sim, graph_steps=8)
     assert (solver.engine.tables.tile_class == 0).any()
     solver.advance(41)
     O.run(case, 41)
@@ -146,7 +151,7 @@ def test_cavity_128_long_vs_cpp(ref_lib):
     case = oracle_case(sim)
     r = ref_lib.RefRunner(case)
     r.initialize()
-    solver = _engine(sim, tile=(128, 2))
+    solver = _engine(sim)
     n = 1000
     solver.advance(n)
     r.run(n)

@@ -1,0 +1,74 @@
+"""Reduced-precision forms against the FP64 oracle.
+
+FP32 compute with FP32, FP16 or BF16 storage of (f - w).  The initial
+velocity field varies from cell to cell, so populations of neighbouring
+cells differ by ~1e-3 and any indexing mistake in the wide-block (2 and 4
+cells per lane) paths shows far above the rounding tolerances:
+  f32: max |f - f_oracle| <= 2e-6, f16dev: <= 1e-4, bf16dev: <= 1e-3
+after 20 steps (rounding of the storage format plus FP32 arithmetic).
+"""
+import numpy as np
+import pytest
+
+from oracle import d2q9_oracle as O
+from pylabolt_amd import cases
+from tests.helpers import oracle_case
+
+pytestmark = pytest.mark.gpu
+
+ABS_TOL = {"f32": 2e-6, "f16dev": 1e-4, "bf16dev": 1e-3}
+
+
+def _rough(sim):
+    sim.initial_fields_dict["default"]["fluid"]["velocity"] = {
+        "type": "func",
+        "func": lambda i, j: (0.03 * np.sin(1.3 * i + 2.1 * j),
+                              0.03 * np.cos(0.7 * i - 1.9 * j))}
+    return sim
+
+
+CASES = {
+    "cavity": lambda: cases.lid_driven_cavity(70, 300),
+    "channel_mrt": lambda: cases.poiseuille_channel(67, 290),
+    "cylinder": lambda: cases.cylinder(96, 530, diameter=20, xc=30),
+}
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16dev", "bf16dev"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_reduced_precision_close_to_oracle(name, storage):
+    from pylabolt_amd.solvers.fluidLB import Solver
+    sim = _rough(CASES[name]())
+    if name == "channel_mrt":
+        sim.forcing_dict["gravity"] = [1e-5, 0.0]
+    case = oracle_case(sim)
+    O.initialize_pop(case)
+    s = Solver(simulation=sim, storage=storage, verbose=False, graph_steps=4)
+    s.set_backend()
+    for chunk in (1, 19):
+        s.advance(chunk)
+        O.run(case, chunk)
+        fl = case.fluid
+        eng = s.engine
+        err_pop = float(np.abs(eng.pop()[fl] - case.pop[fl]).max())
+        err_new = float(np.abs(eng.pop_new()[fl] - case.pop_new[fl]).max())
+        rho, vel = eng.moments_host()
+        err_u = float(np.abs(vel[fl] - case.velocity[fl]).max())
+        print(f"[precision] {name} {storage} step {case.time_step}: "
+              f"pop {err_pop:.2e} pop_new {err_new:.2e} u {err_u:.2e}",
+              flush=True)
+        tol = ABS_TOL[storage]
+        assert err_pop <= tol and err_new <= tol and err_u <= 10 * tol
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16dev", "bf16dev"])
+def test_reduced_precision_mass_fp64(storage):
+    """FP64 mass of the decoded populations equals the host sum."""
+    from pylabolt_amd.solvers.fluidLB import Solver
+    sim = cases.lid_driven_cavity(130, 70)
+    s = Solver(simulation=sim, storage=storage, verbose=False)
+    s.set_backend()
+    s.advance(10)
+    m = s.engine.mass()
+    host = s.engine.pop().sum()
+    assert abs(m - host) <= 1e-9 * abs(host)

@@ -1,0 +1,73 @@
+"""Host-built cell tables: flags, boundary rules, tile classes, fix-up list
+and bulk masks (no GPU)."""
+import numpy as np
+import pytest
+
+from pylabolt_amd import cases
+from pylabolt_amd import engine as E
+from pylabolt_amd.base.state import State
+
+CASES = {
+    "cylinder": lambda: cases.cylinder(200, 300, diameter=30, xc=60),
+    "cavity": lambda: cases.lid_driven_cavity(70, 140),
+    "channel": lambda: cases.poiseuille_channel(33, 260),
+}
+
+
+def _tables(name, B):
+    st = State(CASES[name]())
+    nx, ny = (int(v) for v in st.mesh.grid_global_shape)
+    t = E.CaseTables(nx, ny, st.fields.solid,
+                     E.elements_from_boundary(st.boundary),
+                     st.boundary.x_periodic, st.boundary.y_periodic, B, 4)
+    return st, t, nx, ny
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("B", [64, 128, 256])
+def test_bulk_masks_match_fix_cells(name, B):
+    st, t, nx, ny = _tables(name, B)
+    pitch, _ = E.geometry(nx, ny)
+    fix = set(int(c) for c in t.fix_cells[:t.n_fix])
+    seen = set()
+    ntj = t.ntj
+    masks = t.bulk_masks.reshape(-1, 4, B // 64) if t.n_bulk else None
+    for p in range(t.n_bulk):
+        tid = int(t.tile_list[p])
+        tj, ti = tid % ntj, tid // ntj
+        for r in range(4):
+            for w in range(B // 64):
+                word = int(masks[p, r, w])
+                for b in range(64):
+                    if (word >> b) & 1:
+                        i = 1 + ti * 4 + r
+                        j = 1 + tj * B + 64 * w + b
+                        seen.add(E.ORIGIN + i * pitch + j)
+    assert seen == fix
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_tile_classes_partition(name):
+    st, t, nx, ny = _tables(name, 64)
+    assert t.n_bulk + t.n_general == t.ntj * t.nti
+    assert sorted(t.tile_list.tolist()) == list(range(t.ntj * t.nti))
+    fluid = t.flags_pad[1:-1, 1:-1] & 1
+    for p in range(t.n_bulk):
+        tid = int(t.tile_list[p])
+        tj, ti = tid % t.ntj, tid // t.ntj
+        blk = fluid[ti * 4:ti * 4 + 4, tj * 64:tj * 64 + 64]
+        assert blk.shape == (4, 64) and blk.all()
+
+
+def test_cavity_corner_rules_last_element_wins():
+    """The lid is listed last: its moving-wall rule owns directions 4,7,8 of
+    the top corners, the side walls keep their own directions."""
+    st, t, nx, ny = _tables("cavity", 64)
+    from pylabolt_amd import _native as nt
+    top_left = (1, ny)
+    slot = t.bc_slot_pad[top_left]
+    rules = t.bc_nodes[slot, :9]
+    kinds = [int(r) & 15 for r in rules]
+    assert kinds[4] == nt.RULE_VEL and kinds[7] == nt.RULE_VEL
+    assert kinds[8] == nt.RULE_VEL       # lid wins over the left wall
+    assert kinds[1] == nt.RULE_BB and kinds[5] == nt.RULE_BB
