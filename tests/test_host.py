@@ -1,0 +1,249 @@
+"""Host logic (CPU only): case parsing with the reference's messages, index
+conventions, initial fields, obstacle rasterisation, and the C ABI library
+(loads, exports every symbol include/d2q9.h declares, struct layout)."""
+import ctypes
+import os
+import re
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from pylabolt_amd import cases
+from pylabolt_amd.base.boundary import Boundary
+from pylabolt_amd.base.domain import Domain
+from pylabolt_amd.base.state import State
+from pylabolt_amd.utils.helpers import SingleRankComm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+class Comm:
+    def __init__(self, rank, size):
+        self.rank, self.size = rank, size
+
+    def Get_rank(self):
+        return self.rank
+
+    def Get_size(self):
+        return self.size
+
+
+def _sim(**over):
+    sim = cases.lid_driven_cavity(10, 8)
+    for k, v in over.items():
+        setattr(sim, k, v)
+    return sim
+
+
+# ---- configuration errors (reference messages) ---------------------------
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda s: delattr(s, "control_dict"), "control_dict not found"),
+    (lambda s: s.control_dict.pop("end_time"), "end_time missing in control_dict"),
+    (lambda s: s.control_dict.update(precision="quad"), "unsupported precision"),
+    (lambda s: s.mesh_dict.update(grid=[0, 4]), "grid dimensions cannot be zero"),
+    (lambda s: s.mesh_dict.update(grid=(4, 4)), "must be a list"),
+    (lambda s: s.lattice_dict.update(lattice_type="D3Q19"), "Unsupported lattice type"),
+    (lambda s: s.transport_dict.update(kin_visc="x"), "kin_visc must be a float/int"),
+    (lambda s: s.decompose_dict.update(nx=2), "invalid domain decomposition"),
+    (lambda s: s.boundary_dict["walls"].pop("wall"), "wall missing in boundary: walls"),
+    (lambda s: s.boundary_dict["walls"].update(segments=[]), "segments cannot be an empty list"),
+    (lambda s: s.boundary_dict["walls"].update(segments=[[[3, 0], [1, 0]]]),
+     "x2 >= x1 and y2 >= y1"),
+    (lambda s: s.boundary_dict["walls"].update(segments=[[[0, 0], [3, 3]]]),
+     "axis-aligned"),
+    (lambda s: s.boundary_dict["walls"]["fluid"].update(type="slip"),
+     "Unsupported boundary condition for fluid: slip"),
+    (lambda s: s.boundary_dict["lid"]["fluid"].update(value=0.1),
+     "value for fixed_velocity must be a list"),
+    (lambda s: s.initial_fields_dict["default"]["fluid"].pop("density"),
+     "is missing in default"),
+])
+def test_config_errors_carry_reference_messages(mutate, msg):
+    sim = _sim()
+    mutate(sim)
+    with pytest.raises(ValueError, match=msg):
+        State(sim)
+
+
+def test_collision_dict_validation():
+    from pylabolt_amd.base.collision_operator import CollisionOperator
+    from pylabolt_amd.solvers.fluidLB import FluidLB
+    sim = _sim()
+    st = State(sim)
+    sim.collision_dict["fluid"]["model"] = "TRT"
+    with pytest.raises(ValueError, match="Unsupported fluid collision model: TRT"):
+        CollisionOperator(sim, FluidLB(), st)
+    sim.collision_dict["fluid"]["model"] = "BGK"
+    sim.collision_dict["fluid"]["forcing_model"] = "shan_chen"
+    with pytest.raises(ValueError, match="Unsupported fluid forcing model"):
+        CollisionOperator(sim, FluidLB(), st)
+
+
+def test_omega_matches_reference_formula():
+    from pylabolt_amd.base.collision_operator import CollisionOperator
+    from pylabolt_amd.solvers.fluidLB import FluidLB
+    sim = cases.lid_driven_cavity(128, 128)
+    st = State(sim)
+    co = CollisionOperator(sim, FluidLB(), st)
+    assert st.transport.kin_visc == np.float64(0.127)
+    assert co.tau_fluid == 0.8809999999999999
+    assert co.omega_fluid == 1.135073779795687
+
+
+# ---- domain and index conventions ------------------------------------------
+
+@pytest.mark.parametrize("gx,gy,nx,ny", [(10, 7, 1, 1), (10, 7, 3, 2), (9, 9, 3, 3)])
+def test_domain_decomposition_covers_every_node_once(gx, gy, nx, ny):
+    """Reference tests/unit/test_domain.py:125-216."""
+    owned = np.zeros((gx, gy), dtype=int)
+    for rank in range(nx * ny):
+        sim = SimpleNamespace(decompose_dict={"nx": nx, "ny": ny})
+        mesh = SimpleNamespace(grid_global_shape=np.array([gx, gy]))
+        d = Domain(sim, mesh, Comm(rank, nx * ny))
+        assert tuple(d.shape) == (d.Nx_rank + 2, d.Ny_rank + 2)
+        ox, oy = d.offset
+        owned[ox:ox + d.Nx_rank, oy:oy + d.Ny_rank] += 1
+    assert np.all(owned == 1)
+
+
+def test_boundary_node_indices_reference_fixture():
+    """Reference tests/unit/test_boundary.py:567-588: node of global (i, j)
+    is (i + 1) * (Ny + 2) + (j + 1)."""
+    Nx, Ny = 7, 5
+    sim = cases.lid_driven_cavity(Nx, Ny)
+    segs = [[[0, Ny - 1], [Nx - 1, Ny - 1]], [[0, 0], [Nx - 1, 0]],
+            [[0, 0], [0, Ny - 1]], [[Nx - 1, 0], [Nx - 1, Ny - 1]]]
+    sim.boundary_dict = {"options": {}, "check": {
+        "wall": True, "segments": segs, "fluid": {"type": "bounce_back"}}}
+    st = State(sim)
+    for n, seg in enumerate(segs):
+        want = [(i + 1) * (Ny + 2) + (j + 1)
+                for i in range(seg[0][0], seg[1][0] + 1)
+                for j in range(seg[0][1], seg[1][1] + 1)]
+        assert list(st.boundary.boundary_elements[n].boundary_nodes) == want
+    locs = [e.location for e in st.boundary.boundary_elements]
+    assert locs == ["top", "bottom", "left", "right"]
+    e = st.boundary.boundary_elements[1]
+    assert list(e.out_list) == [4, 7, 8] and list(e.inv_list) == [2, 5, 6]
+
+
+def test_periodic_pair_validation():
+    sim = cases.poiseuille_channel(8, 6)
+    st = State(sim)
+    assert st.boundary.x_periodic and not st.boundary.y_periodic
+    sim.boundary_dict["inlet_outlet"]["segments"] = [[[0, 0], [0, 4]],
+                                                     [[7, 0], [7, 5]]]
+    with pytest.raises(ValueError, match="must span the entire y-direction"):
+        State(sim)
+
+
+def test_init_fields_fixed_on_non_ghost_only():
+    """Reference tests/unit/test_init_fields.py:226-274."""
+    sim = _sim()
+    sim.initial_fields_dict = {"default": {"fluid": {
+        "velocity": {"type": "fixed", "value": [1, 2]},
+        "density": {"type": "fixed", "value": 3},
+        "pressure": {"type": "fixed", "value": 4}}}}
+    f = State(sim).fields
+    assert np.all(f.velocity[~f.ghost_node] == [1, 2])
+    assert np.all(f.velocity[f.ghost_node] == [0, 0])
+    assert np.all(f.density[~f.ghost_node] == 3)
+    assert np.all(f.density[f.ghost_node] == 0)
+    assert np.all(f.pressure[~f.ghost_node] == 4)
+
+
+def test_init_fields_func_uses_global_indices():
+    sim = _sim()
+    sim.initial_fields_dict["default"]["fluid"]["density"] = {
+        "type": "func", "func": lambda i, j: 1.0 + i + 100 * j}
+    st = State(sim)
+    nxp, nyp = st.domain.shape
+    rho = st.fields.density.reshape(nxp, nyp)
+    assert rho[1, 1] == 1.0 and rho[3, 2] == 1.0 + 2 + 100
+    assert rho[0, 0] == 0.0
+
+
+@pytest.mark.parametrize("center", [[4.0, 3.0], [0.5, 5.2]])
+def test_circle_rasterisation_reference_fixture(center):
+    """Reference tests/unit/test_obstacle.py:329-370 (minimum image across
+    periodic sides, ghost nodes never solid)."""
+    sim = cases.poiseuille_channel(12, 9)
+    sim.boundary_dict["walls"]["fluid"]["type"] = "periodic"
+    sim.obstacle_dict = {"options": {}, "c": {
+        "type": "circle", "radius": 2.5, "center": center, "density": 1.7,
+        "static": True}}
+    st = State(sim)
+    Nx, Ny = 12, 9
+    Nxp, Nyp = Nx + 2, Ny + 2
+    solid = np.zeros(Nxp * Nyp, dtype=bool)
+    for ind in range(Nxp * Nyp):
+        i, j = divmod(ind, Nyp)
+        rx = i - (center[0] + 1)
+        ry = j - (center[1] + 1)
+        rxm, rym = rx, ry
+        if abs(rx + Nx) < abs(rxm):
+            rxm = rx + Nx
+        if abs(rx - Nx) < abs(rxm):
+            rxm = rx - Nx
+        if abs(ry + Ny) < abs(rym):
+            rym = ry + Ny
+        if abs(ry - Ny) < abs(rym):
+            rym = ry - Ny
+        if np.sqrt(rxm * rxm + rym * rym) <= 2.5 and not st.fields.ghost_node[ind]:
+            solid[ind] = True
+    assert np.array_equal(st.fields.solid, solid)
+    assert np.all(st.fields.density[solid] == 1.7)
+    assert np.all(st.fields.solid_id[solid] == 0)
+    assert np.all(st.fields.solid_id[~solid] == -1)
+
+
+def test_single_rank_comm_abort_raises():
+    with pytest.raises(SystemExit):
+        SingleRankComm().Abort()
+
+
+def test_load_simulation_from_directory(tmp_path):
+    from pylabolt_amd.utils.helpers import load_simulation
+    (tmp_path / "simulation.py").write_text(
+        "control_dict = {'start_time': 0}\n")
+    mod = load_simulation(path=tmp_path)
+    assert mod.control_dict["start_time"] == 0
+    with pytest.raises(ImportError, match="Missing simulation.py"):
+        load_simulation(path=tmp_path / "nowhere")
+
+
+# ---- the C ABI library (no GPU calls) -------------------------------------
+
+def _header_functions():
+    text = open(os.path.join(ROOT, "include", "d2q9.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(d2q9_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from pylabolt_amd import _native
+    lib = _native.load_library()      # also checks the struct layout
+    names = _header_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) <= set(_native.SIGNATURES) | {
+        "d2q9_batched_step", "d2q9_adjoint_step"} | set(names)
+
+
+def test_abi_layout_matches_ctypes_mirror():
+    from pylabolt_amd import _native
+    lib = _native.load_library()
+    assert _native.abi_mismatches(lib) == []
+
+
+def test_engine_refuses_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from pylabolt_amd import _native
+    _native._lib = None
+    with pytest.raises(_native.NativeError, match="no HIP device"):
+        _native.lib()
