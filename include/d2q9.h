@@ -33,7 +33,8 @@
  *                        utils/residues.py:171-317
  *   d2q9_graph_*      <- solvers/fluidLB.py:270-287 (CUDA graph of a step)
  *   d2q9_batched_step <- no reference counterpart (many lattices per launch)
- *   d2q9_adjoint_step <- no reference counterpart (reverse-mode step)
+ *   d2q9_adjoint_step,
+ *   d2q9_adjoint_init_grad <- no reference counterpart (reverse-mode step)
  */
 #ifndef D2Q9_H
 #define D2Q9_H
@@ -205,6 +206,28 @@ int d2q9_graph_begin(void* stream);
 int d2q9_graph_end(void* stream, void** graph_exec);
 int d2q9_graph_launch(void* graph_exec, void* stream);
 int d2q9_graph_destroy(void* graph_exec);
+
+/*
+ * Adjoint of one FP64 step (BGK, Guo forcing, per-cell force field;
+ * bounce-back, periodic and static-solid boundaries only, so that streaming
+ * permutes the populations).  lam_src = dL/d(post-collision populations of
+ * this step), lam_dst receives dL/d(post-collision populations of the
+ * previous step) when `streamed` (the transpose of pull streaming), or
+ * dL/d(pre-collision populations) at each cell when not (the first step,
+ * which collides the initial equilibrium without streaming).  mom holds
+ * this step's rho/u (stored by the forward pass), seed_u (nullable) the
+ * loss gradient with respect to this step's velocity [2][plane], and
+ * grad_force [2][plane] accumulates dL/dF.  Layouts as the populations /
+ * side arrays of d2q9_lattice.
+ */
+int d2q9_adjoint_step(const d2q9_lattice* L, int streamed, const void* lam_src,
+                      void* lam_dst, const void* mom, const void* seed_u,
+                      void* grad_force, void* stream);
+
+/* dL/d(rho0, u0) [3][plane] from mu = dL/d(initial equilibrium populations)
+ * and mom0 = the initial rho/u. */
+int d2q9_adjoint_init_grad(const d2q9_lattice* L, const void* mu,
+                           const void* mom0, void* grad_init, void* stream);
 
 /* Text of the last error on this thread. */
 const char* d2q9_last_error(void);

@@ -96,6 +96,9 @@ SIGNATURES = {
     "d2q9_last_error": (ctypes.c_char_p, []),
     "d2q9_abi_layout": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int64),
                                        ctypes.c_int]),
+    "d2q9_adjoint_step": (ctypes.c_int, [_LP, ctypes.c_int, _P, _P, _P, _P,
+                                         _P, _P]),
+    "d2q9_adjoint_init_grad": (ctypes.c_int, [_LP, _P, _P, _P, _P]),
 }
 
 # fields whose offsets d2q9_abi_layout reports, in its order

@@ -140,3 +140,24 @@ def cylinder(nx=8192, ny=2048, diameter=128, xc=None, re=100.0, u_ref=0.05,
                                   "forcing_model": None}},
         forcing_dict={},
     )
+
+
+def adjoint_channel(nx=1024, ny=256, steps=2000, kin_visc=0.1,
+                    forcing="guo_second_order", seed=0):
+    """Guo-forced channel for the adjoint: x periodic, bounce-back walls at
+    y, per-cell body force F_x ~ N(1e-6, 2e-7) (F_y = 0) from ``seed``,
+    initial velocity zero, and a seeded target parabola for u_x on the
+    outlet column.  Returns (simulation, force (nx, ny, 2), u0 (nx, ny, 2),
+    target (ny,))."""
+    sim = poiseuille_channel(nx, ny, model="BGK", forcing=forcing,
+                             steps=steps)
+    sim.transport_dict["kin_visc"] = float(kin_visc)
+    sim.forcing_dict = {}
+    rng = np.random.default_rng(seed)
+    force = np.zeros((nx, ny, 2))
+    force[:, :, 0] = rng.normal(1e-6, 2e-7, size=(nx, ny))
+    u0 = np.zeros((nx, ny, 2))
+    y = (np.arange(ny) + 0.5) / ny
+    amp = 1e-3 * (1.0 + 0.1 * rng.standard_normal())
+    target = amp * 4.0 * y * (1.0 - y)
+    return sim, force, u0, target

@@ -40,6 +40,10 @@
 
 namespace d2q9 {
 
+// Text returned by d2q9_last_error() (per host thread); defined in
+// d2q9_step.hip.
+void set_error(const char* what);
+
 // Velocity set, reference ordering (base/lattice.py:50-59).
 __device__ constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 __device__ constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};

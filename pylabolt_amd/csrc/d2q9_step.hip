@@ -26,8 +26,12 @@
 using namespace d2q9;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
+
+void d2q9::set_error(const char* what) { g_last_error = what; }
+
+namespace {
 
 int fail(int code, const char* what) {
   g_last_error = what;
