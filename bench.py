@@ -121,6 +121,54 @@ def cpu_baseline(sim, target_s=15.0, max_steps=200):
                       f"warm-up steps, OpenMP threads={threads}"}
 
 
+def adjoint_cost(steps=2000, reps=3):
+    """Adjoint workload (Guo-forced channel 1024 x 256, FP64, 2000 steps):
+    time of forward + backward over the plain forward of the same steps."""
+    import torch
+    from pylabolt_amd import cases
+    from pylabolt_amd.adjoint import FlowRollout
+    from pylabolt_amd.solvers.fluidLB import Solver
+    sim, F, u0, target = cases.adjoint_channel(steps=steps)
+    nx, ny = F.shape[:2]
+    out = {"workload": f"adjoint channel BGK+Guo FP64 {nx}x{ny}, {steps} steps"}
+    # plain forward: the same lattice, force field, graph-replayed steps
+    from pylabolt_amd.engine import D2Q9Engine, elements_from_boundary
+    ro = {m: FlowRollout(sim, steps, checkpoint=m) for m in ("sqrt", "moments")}
+    eng = ro["sqrt"].eng
+    Ft = torch.tensor(F, device="cuda")
+    ut = torch.tensor(u0, device="cuda")
+    tt = torch.tensor(target, device="cuda")
+
+    def timed(fn):
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
+    def plain():
+        eng.initialize()
+        eng.advance(steps, write_moments=True)
+        eng.synchronize()
+    t_plain = timed(plain)
+    out["plain_forward_ms"] = t_plain * 1e3
+    for m, r in ro.items():
+        u = r.forward(Ft, ut)
+        g = torch.zeros_like(u)
+        g[-1, :, 0] = u[-1, :, 0] - tt
+        t_f = timed(lambda: r.forward(Ft, ut))
+        t_b = timed(lambda: r.backward(g))
+        out[m] = {"forward_ms": t_f * 1e3, "backward_ms": t_b * 1e3,
+                  "cost_ratio": (t_f + t_b) / t_plain,
+                  "checkpoints": len(r.ck), "segment": r.K}
+        r.close()
+    return out
+
+
 def load_pmc(workload, storage):
     path = os.path.join(ROOT, "profiles", f"pmc_{workload}_{storage}.json")
     if os.path.exists(path):
@@ -139,6 +187,18 @@ def run_form(sim, storage, steps, warmup, tile):
 
 
 def main():
+    # everything but the final JSON line goes to stderr
+    real_stdout = sys.stdout
+    sys.stdout = sys.stderr
+    try:
+        out = run(real_stdout)
+    finally:
+        sys.stdout = real_stdout
+    if out is not None:
+        print(json.dumps(out), flush=True)
+
+
+def run(real_stdout):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
@@ -152,12 +212,14 @@ def main():
                     help="extra storage forms to time ('' for none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--profile-run", action="store_true",
                     help="a short run for rocprofv3: main kernel + copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must "
-                          "be 1 (multi-GPU sharding is a design note only)"}))
+                          "be 1 (multi-GPU sharding is a design note only)"}),
+              file=real_stdout)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
@@ -173,7 +235,7 @@ def main():
                                     args.warmup, tile)
         bw = copy_bandwidth(1 << 30, reps=5)
         log(f"profile run: {ms / args.steps * 1e3:.1f} us/step, copy {bw:.0f} GB/s")
-        return
+        return None
 
     log(f"workload {args.workload} {nx}x{ny} storage={args.storage} "
         f"tile={tile} steps={args.steps} warmup={args.warmup}")
@@ -216,6 +278,13 @@ def main():
         del s2
         torch.cuda.empty_cache()
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_cost()
+        log(f"adjoint: plain fwd {adj['plain_forward_ms']:.1f} ms, sqrt "
+            f"ratio {adj['sqrt']['cost_ratio']:.2f}, moments ratio "
+            f"{adj['moments']['cost_ratio']:.2f}")
+        torch.cuda.empty_cache()
     copy_gbs = None if args.no_copy else copy_bandwidth()
     if copy_gbs:
         log(f"copy kernel: {copy_gbs:.0f} GB/s")
@@ -250,11 +319,12 @@ def main():
                      "bytes_per_cell_update": bpc},
         "cpu_baseline": cpu,
         "forms": forms,
+        "adjoint": adj,
         "copy_kernel_gbs": copy_gbs,
     }
     if copy_gbs:
         out["roofline"]["frac_of_copy"] = achieved / copy_gbs
-    print(json.dumps(out))
+    return out
 
 
 if __name__ == "__main__":

@@ -58,7 +58,7 @@ class FlowRollout:
             raise ValueError("checkpoint must be 'sqrt' or 'moments'")
         self.mode = checkpoint
         st = State(simulation)
-        co = CollisionOperator(simulation, FluidLB(), st)
+        co = CollisionOperator(simulation, FluidLB(), st, verbose=False)
         if co.collision_fluid != "BGK":
             raise ValueError("the adjoint supports the BGK collision model")
         for e in st.boundary.boundary_elements:

@@ -82,10 +82,11 @@ class Solver:
                                graph_steps=graph_steps)
         self.mpi_operator = MPIOperator(comm, self.state)
         self.collision_operator = CollisionOperator(simulation, self.model,
-                                                    self.state, comm)
+                                                    self.state, comm,
+                                                    verbose=verbose)
         self.force_operator = ForceOperator(
             simulation, self.model, self.state, comm,
-            collision_operator=self.collision_operator)
+            collision_operator=self.collision_operator, verbose=verbose)
         self.compute_fields_operator = ComputeFieldsOperator(self.model,
                                                              self.state)
         self.streaming_operator = StreamingOperator(self.model, self.state,

@@ -12,8 +12,9 @@ def oracle_case(sim, force_field=None):
     """OracleCase of a simulation namespace (the package builds the case
     arrays; the oracle runs the steps)."""
     st = State(sim)
-    co = CollisionOperator(sim, FluidLB(), st)
-    fo = ForceOperator(sim, FluidLB(), st, collision_operator=co)
+    co = CollisionOperator(sim, FluidLB(), st, verbose=False)
+    fo = ForceOperator(sim, FluidLB(), st, collision_operator=co,
+                       verbose=False)
     return O.case_from_state(
         st, co.collision_fluid, co.forcing_fluid, co.omega_fluid, fo.gravity,
         force_source="gravity" if force_field is None else "field",
