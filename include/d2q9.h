@@ -32,7 +32,8 @@
  *   d2q9_residues     <- parallel/cpu/compute_residues_kernels.py:6-73,
  *                        utils/residues.py:171-317
  *   d2q9_graph_*      <- solvers/fluidLB.py:270-287 (CUDA graph of a step)
- *   d2q9_batched_step <- no reference counterpart (many lattices per launch)
+ *   d2q9_batched_step <- no reference counterpart (many lattices per launch;
+ *                        each lattice's step is d2q9_fused_step)
  *   d2q9_adjoint_step,
  *   d2q9_adjoint_init_grad <- no reference counterpart (reverse-mode step)
  */
@@ -206,6 +207,45 @@ int d2q9_graph_begin(void* stream);
 int d2q9_graph_end(void* stream, void** graph_exec);
 int d2q9_graph_launch(void* graph_exec, void* stream);
 int d2q9_graph_destroy(void* graph_exec);
+
+/*
+ * Batched sweep: many lattices advance one fused step in one persistent
+ * launch.  Every entry carries its lattice descriptor and its two
+ * population and moment buffers; at parity p a lattice streams from f[p]
+ * into f[1-p] (moments mom[p] -> mom[1-p]).  All entries share one storage
+ * type (FP64 or FP32; others return D2Q9_E_UNSUPPORTED) and the workgroup
+ * shape 64 x tile_i; each may have its own size,
+ * boundaries, collision model and forcing.
+ *
+ * Work items (lattice, code): code >= 0 is position `code` of that
+ * lattice's tile list (a bulk tile if code < n_bulk, else a general tile);
+ * code < 0 is the chunk of fix-up cells starting at 64 * tile_i * (-code-1).
+ * The items of one step write disjoint cells, so they run in any order.
+ * queue_bounds[0..8]: XCD x (workgroup b lives on XCD b % 8) owns items
+ * [queue_bounds[x], queue_bounds[x+1]).  policy 0: static, workgroup r of
+ * an XCD takes that XCD's items r, r + grid/8, ...; policy 1: dynamic, each
+ * workgroup claims `chunk` items at a time from its own XCD queue with an
+ * atomic counter, then helps the other queues.  counters: 16 int32, zero before the first
+ * step; a step of parity p uses counters[8p .. 8p+8) and clears the other
+ * half for the next step.  grid must be a multiple of 8.  Entries, items,
+ * bounds and counters are device memory.
+ */
+typedef struct d2q9_batch_entry {
+  d2q9_lattice L;
+  void* f[2];
+  void* mom[2];
+} d2q9_batch_entry;
+
+typedef struct d2q9_batch_item {
+  int32_t lattice;
+  int32_t code;
+} d2q9_batch_item;
+
+int d2q9_batched_step(const d2q9_batch_entry* entries,
+                      const d2q9_batch_item* items,
+                      const int32_t* queue_bounds, int32_t* counters,
+                      int parity, int storage, int tile_i, int grid,
+                      int policy, int chunk, int write_moments, void* stream);
 
 /*
  * Adjoint of one FP64 step (BGK, Guo forcing, per-cell force field;

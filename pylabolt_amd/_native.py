@@ -74,6 +74,17 @@ class Lattice(ctypes.Structure):
     ]
 
 
+class BatchEntry(ctypes.Structure):
+    """Mirror of ``d2q9_batch_entry``."""
+    _fields_ = [("L", Lattice), ("f", ctypes.c_void_p * 2),
+                ("mom", ctypes.c_void_p * 2)]
+
+
+class BatchItem(ctypes.Structure):
+    """Mirror of ``d2q9_batch_item``."""
+    _fields_ = [("lattice", ctypes.c_int32), ("code", ctypes.c_int32)]
+
+
 assert ctypes.sizeof(BcElem) == 40 and ctypes.sizeof(BcNode) == 20
 
 _P = ctypes.c_void_p
@@ -99,6 +110,8 @@ SIGNATURES = {
     "d2q9_adjoint_step": (ctypes.c_int, [_LP, ctypes.c_int, _P, _P, _P, _P,
                                          _P, _P]),
     "d2q9_adjoint_init_grad": (ctypes.c_int, [_LP, _P, _P, _P, _P]),
+    "d2q9_batched_step": (ctypes.c_int, [_P, _P, _P, _P] + [ctypes.c_int] * 7
+                          + [_P]),
 }
 
 # fields whose offsets d2q9_abi_layout reports, in its order
@@ -117,6 +130,8 @@ def abi_mismatches(handle):
     want = [ctypes.sizeof(Lattice), ctypes.sizeof(BcElem),
             ctypes.sizeof(BcNode)]
     want += [getattr(Lattice, f).offset for f in ABI_FIELDS]
+    want += [ctypes.sizeof(BatchEntry), BatchEntry.f.offset,
+             BatchEntry.mom.offset, ctypes.sizeof(BatchItem)]
     return [(i, g, w) for i, (g, w) in enumerate(zip(got, want)) if g != w] \
         + ([("count", n, len(want))] if n != len(want) else [])
 

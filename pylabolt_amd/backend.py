@@ -13,7 +13,8 @@ from pylabolt_amd.engine import D2Q9Engine, elements_from_boundary
 
 class Backend:
     def __init__(self, comm, state, backend_type="gpu", n_threads=None,
-                 storage="f64", tile=(64, 4), device=None, graph_steps=16):
+                 storage="f64", tile=(64, 4), device=None, graph_steps=16,
+                 stream=None):
         if backend_type != "gpu":
             raise ValueError("Unsupported backend: " + str(backend_type) +
                              " (this engine runs on the GPU only)")
@@ -23,6 +24,7 @@ class Backend:
         self.storage = storage
         self.tile = tile
         self.graph_steps = graph_steps
+        self.stream = stream
         self.lib = nt.lib()
         self.device = torch.device(device) if device else torch.device(
             "cuda", torch.cuda.current_device())
@@ -42,5 +44,6 @@ class Backend:
             state.control.float_min, storage=self.storage,
             force_field=force_field, tile=self.tile,
             rates=collision_operator.rates, dev_scale=dev_scale,
-            device=self.device, graph_steps=self.graph_steps)
+            device=self.device, graph_steps=self.graph_steps,
+            stream=self.stream)
         return self.engine

@@ -12,6 +12,7 @@ import os
 import subprocess
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -32,18 +33,26 @@ def sources():
                   if f.endswith(".hip"))
 
 
-def build(verbose=True, extra=()):
-    """Compile all HIP sources into libd2q9.so; raises on failure."""
-    objs = []
+def _compile(src, extra, verbose):
+    obj = os.path.join(CSRC, os.path.basename(src)[:-4] + ".o")
+    cmd = [HIPCC, *[f for f in FLAGS if f != "-shared"], "-c", src,
+           "-o", obj, *extra]
     t0 = time.time()
-    for src in sources():
-        obj = os.path.join(CSRC, os.path.basename(src)[:-4] + ".o")
-        cmd = [HIPCC, *[f for f in FLAGS if f != "-shared"], "-c", src,
-               "-o", obj, *extra]
-        if verbose:
-            print("[build]", os.path.basename(src), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+    subprocess.run(cmd, check=True)
+    if verbose:
+        print(f"[build] {os.path.basename(src)} {time.time() - t0:.1f}s",
+              flush=True)
+    return obj
+
+
+def build(verbose=True, extra=()):
+    """Compile all HIP sources (in parallel) into libd2q9.so; raises on
+    failure."""
+    t0 = time.time()
+    srcs = sources()
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", "8"))))
+    with ThreadPoolExecutor(jobs) as pool:
+        objs = list(pool.map(lambda s: _compile(s, extra, verbose), srcs))
     tmp = LIB + ".tmp"
     subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC",
                     *objs, "-o", tmp], check=True)

@@ -161,3 +161,29 @@ def adjoint_channel(nx=1024, ny=256, steps=2000, kin_visc=0.1,
     amp = 1e-3 * (1.0 + 0.1 * rng.standard_normal())
     target = amp * 4.0 * y * (1.0 - y)
     return sim, force, u0, target
+
+
+def batch_cases(n=256, seed=0, nx_range=(64, 1024), ny_range=(64, 512),
+                step=32, re_range=(50.0, 400.0), steps=5000,
+                precision="single"):
+    """``n`` independent lattices for a batched sweep, drawn from ``seed``:
+    sizes uniform over the multiples of ``step`` in the ranges, Re uniform
+    in ``re_range``; even ids are lid-driven cavities (BGK), odd ids
+    gravity-driven channels (MRT, second-order Guo forcing)."""
+    rng = np.random.default_rng(seed)
+    xs = np.arange(nx_range[0], nx_range[1] + 1, step)
+    ys = np.arange(ny_range[0], ny_range[1] + 1, step)
+    out = []
+    for k in range(n):
+        nx = int(rng.choice(xs))
+        ny = int(rng.choice(ys))
+        re = float(rng.uniform(*re_range))
+        if k % 2 == 0:
+            sim = lid_driven_cavity(nx, ny, re=re, steps=steps, model="BGK",
+                                    precision=precision)
+        else:
+            sim = poiseuille_channel(nx, ny, re=re, steps=steps, model="MRT",
+                                     forcing="guo_second_order",
+                                     precision=precision)
+        out.append(sim)
+    return out

@@ -1,0 +1,322 @@
+// Tile and cell sweeps shared by the single-lattice kernels
+// (d2q9_step.hip) and the batched persistent kernel (d2q9_batched.hip).
+// See d2q9_step.hip for the launch structure.
+#pragma once
+
+#include "d2q9_device.h"
+
+namespace d2q9 {
+
+enum { PHASE_COLLIDE = 0, PHASE_FUSED = 1, PHASE_STREAM = 2 };
+
+// 8-byte vector of CPL raw elements (one lane's share of one direction).
+template <typename R, int CPL>
+struct alignas(8) Vec8 {
+  R v[CPL];
+};
+
+template <typename R, int CPL>
+__device__ __forceinline__ Vec8<R, CPL> load8(const R* p) {
+  static_assert(sizeof(R) * CPL == 8, "one lane moves 8 bytes");
+  uint64_t w = *reinterpret_cast<const uint64_t*>(p);
+  Vec8<R, CPL> r;
+  __builtin_memcpy(&r, &w, 8);
+  return r;
+}
+
+template <typename R, int CPL>
+__device__ __forceinline__ void store8_nt(R* p, const Vec8<R, CPL>& v) {
+  uint64_t w;
+  __builtin_memcpy(&w, &v, 8);
+  __builtin_nontemporal_store(w, reinterpret_cast<uint64_t*>(p));
+}
+
+// Pre-collision values of one flagged cell: bounce-back off solid
+// neighbours, periodic wrap and boundary rules (streaming_kernels.py:29-47,
+// MPI_kernels.py:34-99, fluid_boundary_kernels.py:5-152).
+template <int S, int PHASE>
+__device__ __forceinline__ void gather_cell(
+    const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K,
+    const typename Codec<S>::raw* __restrict__ fsrc,
+    const typename Codec<S>::T* __restrict__ msrc, int i, int j, int64_t c,
+    uint8_t fl, typename Codec<S>::T (&g)[9]) {
+  if (PHASE == PHASE_COLLIDE) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g[k] = ld<S>(L, K, fsrc, i, j, k);
+  } else {
+    g[0] = ld<S>(L, K, fsrc, i, j, 0);
+#pragma unroll
+    for (int k = 1; k < 9; ++k) g[k] = pull<S>(L, K, fsrc, msrc, i, j, c, fl, k);
+    if (fl & D2Q9_CELL_BC) apply_bc<S>(L, K, fsrc, msrc, i, j, c, g);
+  }
+}
+
+// Coalesced gather of one lane's CPL cells (j0 .. j0+CPL-1) of row i: the
+// nine directions of block tj+1 of the pull rows.  The y-shift is a shifted
+// address when CPL == 1; for CPL > 1 it is a cross-lane shuffle, and the two
+// cells at the ends of the wave's block take their neighbour from one
+// six-lane "halo" load (lanes 0..5 fetch, for directions 2,5,6 the element
+// before the block and for 4,7,8 the element after it) broadcast with
+// readlane.  All ten loads are issued before any is consumed.  Exact for
+// interior cells away from periodic edges; no address leaves the padded row.
+__device__ __forceinline__ uint32_t bits32(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ float from_bits32(uint32_t v) { return __uint_as_float(v); }
+
+template <typename T>
+__device__ __forceinline__ T wave_readlane(T v, int lane);
+template <>
+__device__ __forceinline__ float wave_readlane<float>(float v, int lane) {
+  return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v), lane));
+}
+
+template <int S, int PHASE>
+__device__ __forceinline__ void gather_block(
+    const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K,
+    const typename Codec<S>::raw* __restrict__ fsrc, int i, int tj, int lane,
+    int j0, typename Codec<S>::T (&g)[Codec<S>::CPL][9]) {
+  using C = Codec<S>;
+  using T = typename C::T;
+  using R = typename C::raw;
+  constexpr int CPL = C::CPL;
+  constexpr int B = 64 * CPL;
+  if constexpr (CPL == 1) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int row = (PHASE == PHASE_COLLIDE) ? i : i - CX[k];
+      const int cy = (PHASE == PHASE_COLLIDE) ? 0 : CY[k];
+      const int jc = (j0 <= L.ny) ? j0 - cy : j0;
+      g[0][k] = C::dec(fsrc[pop_index<B>(L, row, jc, k)], k, K);
+    }
+  } else {
+    Vec8<R, CPL> v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int row = (PHASE == PHASE_COLLIDE) ? i : i - CX[k];
+      v[k] = load8<R, CPL>(fsrc + (((int64_t)row * L.nbj + tj + 1) * 9 + k) * B + lane * CPL);
+    }
+    R halo = R();
+    if (PHASE != PHASE_COLLIDE) {
+      // lane h < 6 fetches the end element of direction HK[h]
+      const int h = lane < 6 ? lane : 0;
+      const int kh = h == 0 ? 2 : h == 1 ? 5 : h == 2 ? 6 : h == 3 ? 4 : h == 4 ? 7 : 8;
+      const int cxh = h == 0 ? 0 : h == 1 ? 1 : h == 2 ? -1 : h == 3 ? 0 : h == 4 ? -1 : 1;
+      const int jf = 1 + tj * B;             // first cell of the block
+      int jh = h < 3 ? jf - 1 : jf + B;      // element before / after
+      if (jh > L.ny + 1) jh = jf;            // past the padded row: unused
+      halo = fsrc[pop_index<B>(L, i - cxh, jh, kh)];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int cy = (PHASE == PHASE_COLLIDE) ? 0 : CY[k];
+      T x[CPL];
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) x[q] = C::dec(v[k].v[q], k, K);
+      if (cy == 0) {
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) g[q][k] = x[q];
+      } else {
+        const int h = k == 2 ? 0 : k == 5 ? 1 : k == 6 ? 2 : k == 4 ? 3 : k == 7 ? 4 : 5;
+        const T hv = wave_readlane<T>(C::dec(halo, k, K), h);
+        if (cy == 1) {  // cell j takes j - 1
+          T prev = __shfl_up(x[CPL - 1], 1, 64);
+          g[0][k] = lane == 0 ? hv : prev;
+#pragma unroll
+          for (int q = 1; q < CPL; ++q) g[q][k] = x[q - 1];
+        } else {  // cell j takes j + 1
+          T next = __shfl_down(x[0], 1, 64);
+#pragma unroll
+          for (int q = 0; q < CPL - 1; ++q) g[q][k] = x[q + 1];
+          g[CPL - 1][k] = lane == 63 ? hv : next;
+        }
+      }
+    }
+  }
+}
+
+// Moments + collision (or nothing for PHASE_STREAM) of one cell, encoded
+// into out[.][q]; rho/u stored when asked.
+template <int S, int MODEL, int FORCING, int PHASE>
+__device__ __forceinline__ void finish_cell(
+    const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K,
+    typename Codec<S>::T (&g)[9], typename Codec<S>::raw (&out)[9][Codec<S>::CPL],
+    int q, int64_t c, bool store_mom, typename Codec<S>::T* __restrict__ mdst) {
+  using C = Codec<S>;
+  using T = typename C::T;
+  if constexpr (PHASE == PHASE_STREAM) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) out[k][q] = C::enc(g[k], k, K);
+  } else {
+    T ffx = (T)0, ffy = (T)0;
+    if (L.force_source == D2Q9_FSRC_FIELD) {
+      const T* ff = static_cast<const T*>(L.force_field);
+      ffx = ff[c];
+      ffy = ff[L.plane + c];
+    }
+    T drho, rho, ux, uy, fx, fy;
+    moments<T, FORCING, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
+    collide<T, MODEL, FORCING, C::dev>(g, K, drho, rho, ux, uy, fx, fy);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) out[k][q] = C::enc(g[k], k, K);
+    if (store_mom) {
+      mdst[c] = rho;
+      mdst[L.plane + c] = ux;
+      mdst[2 * L.plane + c] = uy;
+    }
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void store_block(
+    const d2q9_lattice& L, typename Codec<S>::raw* __restrict__ fdst, int i,
+    int tj, int lane, const typename Codec<S>::raw (&out)[9][Codec<S>::CPL]) {
+  using R = typename Codec<S>::raw;
+  constexpr int CPL = Codec<S>::CPL;
+  constexpr int B = 64 * CPL;
+  const int64_t obase = (((int64_t)i * L.nbj + tj + 1) * 9) * B + lane * CPL;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    if constexpr (CPL == 1) {
+      store_nt(out[k][0], fdst + obase + (int64_t)k * B);
+    } else {
+      Vec8<R, CPL> v;
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) v.v[q] = out[k][q];
+      store8_nt<R, CPL>(fdst + obase + (int64_t)k * B, v);
+    }
+  }
+}
+
+// Bulk tile at position p of the tile list (host-classified: every cell
+// fluid and inside the lattice).  Every cell is swept as plain interior
+// fluid and no flag is read; the cells marked in bulk_masks (solid
+// neighbour, boundary rule, stored moment, periodic edge) are not stored
+// here -- the fix-up pass writes them.
+template <int S, int MODEL, int FORCING, int PHASE>
+__device__ __forceinline__ void bulk_tile(
+    const d2q9_lattice& L, const typename Codec<S>::raw* __restrict__ fsrc,
+    typename Codec<S>::raw* __restrict__ fdst,
+    typename Codec<S>::T* __restrict__ mdst, int write_mom, int ntj, int p) {
+  using C = Codec<S>;
+  using T = typename C::T;
+  constexpr int CPL = C::CPL;
+  constexpr int B = 64 * CPL;
+  const int t = L.tiles[p];
+  const int tj = t % ntj, ti = t / ntj;
+  const int lane = (int)threadIdx.x;
+  const int i = 1 + ti * (int)blockDim.y + (int)threadIdx.y;
+  const int j0 = 1 + tj * B + lane * CPL;
+  if (i > L.nx) return;  // whole wave (never for a bulk tile)
+  const Consts<T> K = load_consts<T>(L);
+  // special bits of this lane's cells
+  const uint64_t* mrow = L.bulk_masks + ((int64_t)p * blockDim.y + threadIdx.y) * CPL;
+  uint32_t skip = 0;
+  bool any = false;
+#pragma unroll
+  for (int w = 0; w < CPL; ++w) any |= mrow[w] != 0;
+  if (any) {
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      const int o = lane * CPL + q;
+      skip |= (uint32_t)((mrow[o >> 6] >> (o & 63)) & 1ull) << q;
+    }
+  }
+  T g[CPL][9];
+  gather_block<S, PHASE>(L, K, fsrc, i, tj, lane, j0, g);
+  typename C::raw out[9][CPL];
+#pragma unroll
+  for (int q = 0; q < CPL; ++q)
+    finish_cell<S, MODEL, FORCING, PHASE>(L, K, g[q], out, q, cell_index(L, i, j0 + q),
+                                          write_mom != 0 && !((skip >> q) & 1), mdst);
+  if (skip == 0) {
+    store_block<S>(L, fdst, i, tj, lane, out);
+  } else {
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      if ((skip >> q) & 1) continue;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) fdst[pop_index<B>(L, i, j0 + q, k)] = out[k][q];
+    }
+  }
+}
+
+// Fix-up cells [first, first + workgroup size) of the lattice's list, one
+// thread per cell.
+template <int S, int MODEL, int FORCING, int PHASE>
+__device__ __forceinline__ void fix_chunk(
+    const d2q9_lattice& L, const typename Codec<S>::raw* __restrict__ fsrc,
+    typename Codec<S>::raw* __restrict__ fdst,
+    const typename Codec<S>::T* __restrict__ msrc,
+    typename Codec<S>::T* __restrict__ mdst, int write_mom, int first) {
+  using C = Codec<S>;
+  using T = typename C::T;
+  constexpr int CPL = C::CPL;
+  constexpr int B = 64 * CPL;
+  const int n = first + (int)(threadIdx.y * blockDim.x + threadIdx.x);
+  if (n >= L.n_fix) return;
+  const Consts<T> K = load_consts<T>(L);
+  const int64_t c = L.fix_cells[n];
+  const int i = (int)((c - L.origin) / L.pitch), j = (int)((c - L.origin) % L.pitch);
+  const uint8_t fl = L.flags[c];
+  T g[9];
+  gather_cell<S, PHASE>(L, K, fsrc, msrc, i, j, c, fl, g);
+  typename C::raw out[9][CPL];
+  finish_cell<S, MODEL, FORCING, PHASE>(
+      L, K, g, out, 0, c, write_mom || (fl & D2Q9_CELL_STORE_MOM), mdst);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) fdst[pop_index<B>(L, i, j, k)] = out[k][0];
+}
+
+// General tile t (one holding solid cells or reaching past the lattice):
+// coalesced gather for every lane, the special cells re-gathered one by
+// one, non-fluid cells written back as the encoded zero.
+template <int S, int MODEL, int FORCING, int PHASE>
+__device__ __forceinline__ void general_tile(
+    const d2q9_lattice& L, const typename Codec<S>::raw* __restrict__ fsrc,
+    typename Codec<S>::raw* __restrict__ fdst,
+    const typename Codec<S>::T* __restrict__ msrc,
+    typename Codec<S>::T* __restrict__ mdst, int write_mom, int ntj, int t) {
+  using C = Codec<S>;
+  using T = typename C::T;
+  constexpr int CPL = C::CPL;
+  constexpr int B = 64 * CPL;
+  const int tj = t % ntj, ti = t / ntj;
+  const int lane = (int)threadIdx.x;
+  const int i = 1 + ti * (int)blockDim.y + (int)threadIdx.y;
+  const int j0 = 1 + tj * B + lane * CPL;
+  if (i > L.nx) return;  // whole wave
+  const Consts<T> K = load_consts<T>(L);
+  T g[CPL][9];
+  gather_block<S, PHASE>(L, K, fsrc, i, tj, lane, j0, g);
+  uint8_t fl[CPL];
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const int j = j0 + q;
+    fl[q] = 0;
+    if (j > L.ny) continue;
+    const int64_t c = cell_index(L, i, j);
+    fl[q] = L.flags[c];
+    if (!(fl[q] & D2Q9_CELL_FLUID)) continue;
+    const bool edge = (L.x_periodic && (i == 1 || i == L.nx)) ||
+                      (L.y_periodic && (j == 1 || j == L.ny));
+    if (fl[q] != D2Q9_CELL_FLUID || (edge && PHASE != PHASE_COLLIDE))
+      gather_cell<S, PHASE>(L, K, fsrc, msrc, i, j, c, fl[q], g[q]);
+  }
+  typename C::raw out[9][CPL];
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    if (!(fl[q] & D2Q9_CELL_FLUID)) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) out[k][q] = enc_zero<S>(k, K);
+      continue;
+    }
+    finish_cell<S, MODEL, FORCING, PHASE>(
+        L, K, g[q], out, q, cell_index(L, i, j0 + q),
+        write_mom || (fl[q] & D2Q9_CELL_STORE_MOM), mdst);
+  }
+  // Cells beyond ny in a partial block: the ghost column ny+1 gets its
+  // encoded zero back, the block padding after it is never read.
+  if (CPL == 1 && j0 > L.ny + 1) return;
+  store_block<S>(L, fdst, i, tj, lane, out);
+}
+
+}  // namespace d2q9

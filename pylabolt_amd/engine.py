@@ -237,7 +237,7 @@ class D2Q9Engine:
                  x_periodic, y_periodic, model, forcing, omega, gravity,
                  inv_cs_2, inv_cs_4, weights, float_min, storage="f64",
                  force_field=None, tile=(64, 4), rates=None, dev_scale=None,
-                 device=None, graph_steps=16):
+                 device=None, graph_steps=16, stream=None):
         self.lib = nt.lib()
         self.device = (torch.device(device) if device is not None else
                        torch.device("cuda", torch.cuda.current_device()))
@@ -253,12 +253,10 @@ class D2Q9Engine:
         self.B, self.nbj, self.pop_len = pop_geometry(self.nx, self.ny,
                                                       self.cpl)
         self.tile_i = int(tile[1]) if tile is not None else 4
-        if self.tile_i != 4:
-            raise ValueError("the sweep runs 256-thread workgroups: 4 rows "
-                             "per tile")
         self.tables = CaseTables(self.nx, self.ny, solid_pad, elements,
                                  x_periodic, y_periodic, self.B, self.tile_i)
-        self.stream = torch.cuda.Stream(device=self.device)
+        self.stream = (stream if stream is not None else
+                       torch.cuda.Stream(device=self.device))
         self._stream_ptr = ctypes.c_void_p(self.stream.cuda_stream)
         dev = self.device
         P = self.plane
@@ -281,10 +279,6 @@ class D2Q9Engine:
             raw = bytes(elems)
             self.bc_elems = torch.frombuffer(bytearray(raw),
                                              dtype=torch.uint8).to(dev)
-            self.tile_list = torch.from_numpy(self.tables.tile_list).to(dev)
-            self.fix_cells = torch.from_numpy(self.tables.fix_cells).to(dev)
-            self.bulk_masks = torch.from_numpy(
-                self.tables.bulk_masks.view(np.int64)).to(dev)
             self.force_field = None
             if force_field is not None:
                 self.force_field = self._pitched_vector(force_field)
@@ -328,11 +322,6 @@ class D2Q9Engine:
         cf.float_min = L.float_min
         cf.gravity[0], cf.gravity[1] = L.gravity[0], L.gravity[1]
         L.flags = self.flags.data_ptr()
-        L.tiles = self.tile_list.data_ptr()
-        L.n_bulk, L.n_general = self.tables.n_bulk, self.tables.n_general
-        L.fix_cells = self.fix_cells.data_ptr()
-        L.n_fix = self.tables.n_fix
-        L.bulk_masks = self.bulk_masks.data_ptr()
         if self.tables.n_bc_nodes:
             L.bc_slot = self.bc_slot.data_ptr()
             L.bc_nodes = self.bc_nodes.data_ptr()
@@ -341,12 +330,42 @@ class D2Q9Engine:
                          if self.force_field is not None else None)
         self.L = L
         self._Lp = ctypes.byref(self.L)
+        self._upload_tiles()
         self.graph_steps = max(2, int(graph_steps) // 2 * 2)
         self._graphs = {}
         self.time_step = 0
         self._init_density = np.asarray(density, dtype=np.float64)
         self._init_velocity = np.asarray(velocity, dtype=np.float64)
         self.initialize()
+
+    def _upload_tiles(self):
+        """Copy the tile list, bulk masks and fix-up cells of the current
+        tile shape to the device and point the descriptor at them."""
+        dev = self.device
+        with torch.cuda.stream(self.stream):
+            self.tile_list = torch.from_numpy(self.tables.tile_list).to(dev)
+            self.fix_cells = torch.from_numpy(self.tables.fix_cells).to(dev)
+            self.bulk_masks = torch.from_numpy(
+                self.tables.bulk_masks.view(np.int64)).to(dev)
+        L = self.L
+        L.tile_j, L.tile_i = self.tables.tile_j, self.tables.tile_i
+        L.tiles = self.tile_list.data_ptr()
+        L.n_bulk, L.n_general = self.tables.n_bulk, self.tables.n_general
+        L.fix_cells = self.fix_cells.data_ptr()
+        L.n_fix = self.tables.n_fix
+        L.bulk_masks = self.bulk_masks.data_ptr()
+
+    def set_tile_rows(self, tile_i):
+        """Re-tile with ``tile_i`` rows per tile (workgroup 64 x tile_i);
+        drops captured graphs, which hold the old tile tables."""
+        tile_i = int(tile_i)
+        if tile_i == self.tile_i:
+            return
+        self.stream.synchronize()
+        self.close()
+        self.tables.set_tile(self.B, tile_i)
+        self.tile_i = tile_i
+        self._upload_tiles()
 
     # ---- layout helpers -------------------------------------------------
     def _upload_plane(self, pad, dtype, count=1):

@@ -229,8 +229,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 15
     for name in names:
         assert hasattr(lib, name), name
-    assert set(names) <= set(_native.SIGNATURES) | {
-        "d2q9_batched_step", "d2q9_adjoint_step"} | set(names)
+    assert set(names) == set(_native.SIGNATURES)
 
 
 def test_abi_layout_matches_ctypes_mirror():
