@@ -19,9 +19,13 @@ The CPU baseline is the C++/OpenMP restatement of the reference's
 multi-pass step (oracle/d2q9_ref.cpp) on the same workload for a bounded
 number of steps, on the host cores.
 
-Also reported (``forms``): FP32 and 16-bit-storage runs of the same
+Also reported: ``forms``, FP32 and 16-bit-storage runs of the same
 workload with their roofline fractions and velocity error against the
-FP64 run, and a plain copy kernel's bandwidth on the same device.
+FP64 run (relative L2 over the fluid cells, and RMS error over the case's
+reference speed u_ref = 0.05); ``batched``, 256 seeded lattices of uneven size advanced by one
+persistent launch per step; ``adjoint``, the forward + backward cost of the
+adjoint channel over its plain forward; and a plain copy kernel's bandwidth
+on the same device.
 """
 import argparse
 import json
@@ -38,6 +42,8 @@ BYTES_PER_CELL = {"f64": 144, "f32": 72, "f16dev": 36, "bf16dev": 36}
 DTYPES = {"f64": "f64", "f32": "f32", "f16dev": "f32-compute/f16-store",
           "bf16dev": "f32-compute/bf16-store"}
 PEAK_GBS = 8000.0
+# reference speed of the cylinder case (Re = u_ref D / nu)
+U_REF = 0.05
 
 
 def log(*a):
@@ -55,17 +61,17 @@ def make_workload(name, nx, ny, steps):
     raise ValueError(name)
 
 
-def time_engine(solver, steps, warmup):
+def time_engine(solver, steps, warmup, use_graph=True):
     import torch
     eng = solver.engine
-    eng.advance(warmup, write_moments=False)
+    eng.advance(warmup, write_moments=False, use_graph=use_graph)
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record(eng.stream)
-    eng.advance(steps, write_moments=False)
+    eng.advance(steps, write_moments=False, use_graph=use_graph)
     stop.record(eng.stream)
     stop.synchronize()
     torch.cuda.synchronize()
@@ -169,6 +175,64 @@ def adjoint_cost(steps=2000, reps=3):
     return out
 
 
+def batched_form(steps=5000, warmup=50, per_lattice_steps=100, n=256,
+                 seed=0):
+    """Batched workload: ``n`` seeded lattices (sizes uniform over the
+    multiples of 32 in 64..1024 x 64..512, Re in [50, 400], even ids cavity
+    BGK, odd ids Guo-forced channel MRT), FP32, one persistent launch per
+    step replayed from HIP graphs, launch configuration from the
+    autotuner.  Aggregate MLUPS = sum of cells * steps / t.  The same
+    lattices advanced one launch per lattice per step are timed beside it
+    over ``per_lattice_steps`` steps."""
+    import torch
+    from pylabolt_amd import cases
+    from pylabolt_amd.batch import BatchAutotuner, BatchedLattices
+    sims = cases.batch_cases(n=n, seed=seed, steps=steps)
+    t0 = time.perf_counter()
+    tuner = BatchAutotuner(reps=5)
+    batch = BatchedLattices(sims, storage="f32", tuner=tuner, graph_steps=50)
+    t_setup = time.perf_counter() - t0
+    batch.advance(warmup)
+    batch.synchronize()
+
+    def timed(fn):
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record(batch.stream)
+        fn()
+        ev1.record(batch.stream)
+        ev1.synchronize()
+        return ev0.elapsed_time(ev1) * 1e-3
+
+    t = timed(lambda: batch.advance(steps, write_moments=True))
+    cells = batch.cells
+    mlups = cells * steps / t / 1e6
+    achieved = mlups * 1e6 * BYTES_PER_CELL["f32"] / 1e9
+    finite = all(bool(np.isfinite(e.moments_host()[1]).all())
+                 for e in batch.engines[:8])
+    mlups_pl = None
+    if per_lattice_steps > 0:
+        t_pl = timed(lambda: batch.advance_per_lattice(per_lattice_steps))
+        mlups_pl = cells * per_lattice_steps / t_pl / 1e6
+    out = {
+        "workload": f"{n} lattices (seed {seed}), FP32, {cells} cells, "
+                    f"{steps} steps",
+        "lattices": n, "cells": cells, "steps": steps,
+        "mlups": mlups, "us_per_step": t / steps * 1e6,
+        "achieved_gbs": achieved, "frac": achieved / PEAK_GBS,
+        "config": batch.config.as_dict(), "items_per_step": batch.n_items,
+        "tuner_candidates": len(tuner.trials),
+        "setup_and_tune_s": t_setup,
+        "per_lattice_launches_mlups": mlups_pl,
+        "speedup_vs_per_lattice_launches": (mlups / mlups_pl
+                                            if mlups_pl else None),
+        "velocity_finite": finite,
+    }
+    batch.close()
+    return out
+
+
 def load_pmc(workload, storage):
     path = os.path.join(ROOT, "profiles", f"pmc_{workload}_{storage}.json")
     if os.path.exists(path):
@@ -177,12 +241,12 @@ def load_pmc(workload, storage):
     return None
 
 
-def run_form(sim, storage, steps, warmup, tile):
+def run_form(sim, storage, steps, warmup, tile, use_graph=True):
     from pylabolt_amd.solvers.fluidLB import Solver
     solver = Solver(simulation=sim, storage=storage, tile=tile,
                     graph_steps=32, verbose=False)
     solver.set_backend()
-    ms, wall = time_engine(solver, steps, warmup)
+    ms, wall = time_engine(solver, steps, warmup, use_graph)
     return solver, ms, wall
 
 
@@ -213,8 +277,12 @@ def run(real_stdout):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--profile-run", action="store_true",
-                    help="a short run for rocprofv3: main kernel + copy")
+                    help="a short run for rocprofv3: main kernel + copy "
+                    "(--workload batched: the batched sweep only)")
+    ap.add_argument("--eager", action="store_true",
+                    help="profile run without HIP graphs (for counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must "
@@ -225,6 +293,11 @@ def run(real_stdout):
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
     tile = tuple(int(v) for v in args.tile.split("x"))
+    if args.profile_run and args.workload == "batched":
+        bat = batched_form(args.steps, args.warmup, per_lattice_steps=0)
+        log(f"profile run batched: {bat['us_per_step']:.1f} us/step "
+            f"{bat['mlups']:.0f} MLUPS config {bat['config']}")
+        return None
     sim = make_workload(args.workload, args.nx, args.ny,
                         args.steps + args.warmup)
     nx, ny = sim.mesh_dict["grid"]
@@ -232,7 +305,8 @@ def run(real_stdout):
 
     if args.profile_run:
         solver, ms, wall = run_form(sim, args.storage, args.steps,
-                                    args.warmup, tile)
+                                    args.warmup, tile,
+                                    use_graph=not args.eager)
         bw = copy_bandwidth(1 << 30, reps=5)
         log(f"profile run: {ms / args.steps * 1e3:.1f} us/step, copy {bw:.0f} GB/s")
         return None
@@ -268,16 +342,27 @@ def run(real_stdout):
         du = vel2[fluid] - ref_vel[fluid]
         l2 = float(np.sqrt((du ** 2).sum() / max((ref_vel[fluid] ** 2).sum(),
                                                  1e-300)))
+        rms_ref = float(np.sqrt((du ** 2).sum(axis=-1).mean())) / U_REF
         forms[form] = {"mlups": m2, "us_per_step": st2 * 1e3, "gbs": a2,
                        "frac": a2 / PEAK_GBS,
                        "speedup_vs_" + args.storage: m2 / mlups,
                        "u_rel_l2_vs_" + args.storage: l2,
+                       "u_rms_err_over_u_ref": rms_ref,
+                       "u_rms_" + args.storage: float(np.sqrt(
+                           (ref_vel[fluid] ** 2).sum(axis=-1).mean())),
                        "steps_compared": steps_done}
         log(f"{form}: {st2 * 1e3:.1f} us/step {m2:.0f} MLUPS {a2:.0f} GB/s "
-            f"u L2 rel err {l2:.2e}")
+            f"u L2 rel err {l2:.2e}, rms err / u_ref {rms_ref:.2e}")
         del s2
         torch.cuda.empty_cache()
 
+    bat = None
+    if not args.no_batched:
+        bat = batched_form()
+        log(f"batched: {bat['mlups']:.0f} MLUPS ({bat['frac']:.1%} of peak) "
+            f"config {bat['config']}, per-lattice launches "
+            f"{bat['per_lattice_launches_mlups']:.0f} MLUPS")
+        torch.cuda.empty_cache()
     adj = None
     if not args.no_adjoint:
         adj = adjoint_cost()
@@ -319,6 +404,7 @@ def run(real_stdout):
                      "bytes_per_cell_update": bpc},
         "cpu_baseline": cpu,
         "forms": forms,
+        "batched": bat,
         "adjoint": adj,
         "copy_kernel_gbs": copy_gbs,
     }

@@ -225,7 +225,10 @@ int d2q9_graph_destroy(void* graph_exec);
  * [queue_bounds[x], queue_bounds[x+1]).  policy 0: static, workgroup r of
  * an XCD takes that XCD's items r, r + grid/8, ...; policy 1: dynamic, each
  * workgroup claims `chunk` items at a time from its own XCD queue with an
- * atomic counter, then helps the other queues.  counters: 16 int32, zero before the first
+ * atomic counter, then helps the other queues.  waves: 0 lets the compiler
+ * allocate registers freely; 6 or 8 selects a build of the kernel whose
+ * registers allow that many waves per SIMD (more latency hiding, some
+ * register spills).  counters: 16 int32, zero before the first
  * step; a step of parity p uses counters[8p .. 8p+8) and clears the other
  * half for the next step.  grid must be a multiple of 8.  Entries, items,
  * bounds and counters are device memory.
@@ -245,7 +248,8 @@ int d2q9_batched_step(const d2q9_batch_entry* entries,
                       const d2q9_batch_item* items,
                       const int32_t* queue_bounds, int32_t* counters,
                       int parity, int storage, int tile_i, int grid,
-                      int policy, int chunk, int write_moments, void* stream);
+                      int policy, int chunk, int waves, int write_moments,
+                      void* stream);
 
 /*
  * Adjoint of one FP64 step (BGK, Guo forcing, per-cell force field;

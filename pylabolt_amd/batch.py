@@ -6,13 +6,15 @@ case.  ``BatchedLattices`` keeps one engine per case -- each with its own
 tables, collision model, forcing and boundaries -- and advances all of them
 by one fused step in a single launch of ``d2q9_batched_step``: the step's
 work is a list of items (bulk tiles, general tiles and chunks of fix-up
-cells of every lattice) cut into eight queues, one per XCD, and a fixed grid
-of workgroups walks them statically or claims them with atomic counters.
+cells of every lattice) cut into eight queues, one per XCD (small lattices
+whole, large ones in row-ordered pieces), and a fixed grid of workgroups
+walks them statically or claims them with atomic counters.
 Full chunks of steps replay a captured HIP graph.
 
-``BatchAutotuner`` picks the tile shape (rows per workgroup), the queue
-policy, the grid and the claim size by timing candidates on the batch
-itself, and caches the winner per size class of the batch.
+``BatchAutotuner`` picks the tile shape (rows per workgroup), the register
+budget of the kernel build (waves per SIMD), the queue policy, the grid and
+the claim size by timing candidates on the batch itself, and caches the
+winner per size class of the batch.
 """
 import ctypes
 import hashlib
@@ -50,26 +52,42 @@ def work_items(engines, tile_i):
     return per
 
 
-def pack_queues(engines, per_items, n_queues=N_XCD):
-    """Assign whole lattices to the XCD queues, largest first onto the
-    least-loaded queue (LPT), and concatenate each queue's items.
+def pack_queues(engines, per_items, n_queues=N_XCD, split=4):
+    """Cut the lattices' item lists into pieces and assign the pieces to
+    the XCD queues, largest first onto the least-loaded queue (LPT); each
+    queue is the concatenation of its pieces.
 
-    Returns (items [n, 2] int32, bounds [n_queues + 1] int32, owner list).
+    A lattice whose item list is longer than 1 / (n_queues * split) of the
+    whole batch is cut into contiguous pieces of about that size (its rows
+    stay in order inside a piece); smaller lattices stay whole, so their
+    tiles share one XCD's L2.  The cost of a piece is its item count.
+
+    Returns (items [n, 2] int32, bounds [n_queues + 1] int32, owner) with
+    owner[q] the list of (lattice, first, stop) pieces of queue q.
     """
-    cost = [e.nx * e.ny for e in engines]
-    order = sorted(range(len(engines)), key=lambda l: (-cost[l], l))
+    total = sum(int(a.shape[0]) for a in per_items)
+    cap = max(1, -(-total // (n_queues * split)))
+    pieces = []
+    for l, a in enumerate(per_items):
+        n = int(a.shape[0])
+        k = max(1, -(-n // cap))
+        edges = [n * p // k for p in range(k + 1)]
+        pieces += [(l, edges[p], edges[p + 1]) for p in range(k)
+                   if edges[p + 1] > edges[p]]
+    order = sorted(range(len(pieces)),
+                   key=lambda p: (-(pieces[p][2] - pieces[p][1]), p))
     load = [0] * n_queues
     owner = [[] for _ in range(n_queues)]
-    for l in order:
+    for p in order:
         q = min(range(n_queues), key=lambda x: (load[x], x))
-        owner[q].append(l)
-        load[q] += cost[l]
+        owner[q].append(pieces[p])
+        load[q] += pieces[p][2] - pieces[p][1]
     chunks, bounds = [], [0]
     for q in range(n_queues):
-        for l in owner[q]:
-            chunks.append(per_items[l])
-        bounds.append(bounds[-1] + sum(per_items[l].shape[0]
-                                       for l in owner[q]))
+        owner[q].sort()
+        for l, a, b in owner[q]:
+            chunks.append(per_items[l][a:b])
+        bounds.append(bounds[-1] + sum(b - a for _, a, b in owner[q]))
     items = (np.concatenate(chunks) if chunks else
              np.zeros((0, 2), dtype=np.int32))
     return (np.ascontiguousarray(items, dtype=np.int32),
@@ -77,21 +95,24 @@ def pack_queues(engines, per_items, n_queues=N_XCD):
 
 
 class BatchConfig:
-    """Launch configuration: rows per tile, queue policy, grid, claim."""
+    """Launch configuration: rows per tile, queue policy, grid, claim size,
+    register budget (waves per SIMD; 0 = the compiler's allocation)."""
 
-    def __init__(self, tile_i=4, policy=POLICY_DYNAMIC, grid=None, chunk=1):
+    def __init__(self, tile_i=4, policy=POLICY_DYNAMIC, grid=None, chunk=1,
+                 waves=0):
         self.tile_i, self.policy = int(tile_i), int(policy)
-        self.grid, self.chunk = grid, int(chunk)
+        self.grid, self.chunk, self.waves = grid, int(chunk), int(waves)
 
     def as_dict(self):
         return {"tile_i": self.tile_i,
                 "policy": "dynamic" if self.policy else "static",
-                "grid": self.grid, "chunk": self.chunk}
+                "grid": self.grid, "chunk": self.chunk, "waves": self.waves}
 
     @classmethod
     def from_dict(cls, d):
         return cls(d["tile_i"], POLICY_DYNAMIC if d["policy"] == "dynamic"
-                   else POLICY_STATIC, d["grid"], d["chunk"])
+                   else POLICY_STATIC, d["grid"], d["chunk"],
+                   d.get("waves", 0))
 
     def __repr__(self):
         return f"BatchConfig({self.as_dict()})"
@@ -145,31 +166,34 @@ class BatchedLattices:
         self.set_config(config)
 
     # ---- configuration ----------------------------------------------------
-    def resident_grid(self, tile_i):
-        """Workgroups that fit on the GPU at once (one per 64 x tile_i
-        tile, four waves per SIMD)."""
+    def resident_grid(self, tile_i, waves=0):
+        """Workgroups that fit on the GPU at once: 64 x tile_i threads
+        each, ``waves`` per SIMD (four SIMDs per CU) for the register-capped
+        builds, four for the default build."""
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
-        per_cu = max(1, 16 // tile_i)
+        per_cu = max(1, 4 * (waves or 4) // tile_i)
         return int(cus * per_cu)
 
     def set_config(self, config):
         cfg = BatchConfig(config.tile_i, config.policy, config.grid,
-                          config.chunk)
+                          config.chunk, config.waves)
         if cfg.grid is None:
-            cfg.grid = self.resident_grid(cfg.tile_i)
+            cfg.grid = self.resident_grid(cfg.tile_i, cfg.waves)
         cfg.grid = max(N_XCD, int(cfg.grid) // N_XCD * N_XCD)
         if self.config is not None and cfg.tile_i != self.config.tile_i:
             self.stream.synchronize()
-        for e in self.engines:
-            e.set_tile_rows(cfg.tile_i)
-        items, bounds, self.owner = pack_queues(
-            self.engines, work_items(self.engines, cfg.tile_i))
-        with torch.cuda.stream(self.stream):
-            self.items = torch.from_numpy(items.reshape(-1)).to(self.device)
-            self.bounds = torch.from_numpy(bounds).to(self.device)
-        self.n_items = int(items.shape[0])
-        self.queue_sizes = np.diff(bounds)
-        self._entry_key = None
+        if self.config is None or cfg.tile_i != self.config.tile_i:
+            for e in self.engines:
+                e.set_tile_rows(cfg.tile_i)
+            items, bounds, self.owner = pack_queues(
+                self.engines, work_items(self.engines, cfg.tile_i))
+            with torch.cuda.stream(self.stream):
+                self.items = torch.from_numpy(items.reshape(-1)).to(
+                    self.device)
+                self.bounds = torch.from_numpy(bounds).to(self.device)
+            self.n_items = int(items.shape[0])
+            self.queue_sizes = np.diff(bounds)
+            self._entry_key = None
         self._drop_graphs()
         self.config = cfg
 
@@ -205,7 +229,7 @@ class BatchedLattices:
             nt.ptr(self.entries), nt.ptr(self.items), nt.ptr(self.bounds),
             nt.ptr(self.counters), int(parity),
             self.engines[0].storage, c.tile_i, c.grid, c.policy, c.chunk,
-            int(write_moments), self._sp)
+            c.waves, int(write_moments), self._sp)
 
     def _after_steps(self, k, write_moments):
         flip = k & 1
@@ -327,7 +351,8 @@ class BatchedLattices:
 
 
 class BatchAutotuner:
-    """Chooses tile shape, queue policy, grid and claim size per size class.
+    """Chooses tile shape, register budget, queue policy, grid and claim
+    size per size class.
 
     A batch's size class is its storage form plus the count of lattices per
     (log2 of nx, number of row blocks along y, collision model, forcing)
@@ -336,7 +361,8 @@ class BatchAutotuner:
     """
 
     TILE_ROWS = (4, 2, 1)
-    GRID_FACTORS = (1, 2, 4)
+    WAVES = (0, 6, 8)
+    GRID_FACTORS = (1, 2)
     CHUNKS = (1, 4)
 
     def __init__(self, cache_path=None, reps=5, verbose=False):
@@ -363,11 +389,14 @@ class BatchAutotuner:
     def candidates(self, batch):
         out = []
         for ti in self.TILE_ROWS:
-            base = batch.resident_grid(ti)
-            for gf in self.GRID_FACTORS:
-                out.append(BatchConfig(ti, POLICY_STATIC, base * gf, 1))
-                for ch in self.CHUNKS:
-                    out.append(BatchConfig(ti, POLICY_DYNAMIC, base * gf, ch))
+            for wv in self.WAVES:
+                base = batch.resident_grid(ti, wv)
+                for gf in self.GRID_FACTORS:
+                    out.append(BatchConfig(ti, POLICY_STATIC, base * gf, 1,
+                                           wv))
+                    for ch in self.CHUNKS:
+                        out.append(BatchConfig(ti, POLICY_DYNAMIC, base * gf,
+                                               ch, wv))
         return out
 
     def tune(self, batch):

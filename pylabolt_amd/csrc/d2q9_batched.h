@@ -57,8 +57,10 @@ __device__ __forceinline__ void dispatch_item(const d2q9_batch_entry* __restrict
   }
 }
 
-template <int S>
-__global__ __launch_bounds__(256) void k_batched(const d2q9_batch_entry* __restrict__ entries,
+// W: waves per SIMD the register allocation must allow (1: no limit
+// beyond the compiler's own choice).
+template <int S, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_batched(const d2q9_batch_entry* __restrict__ entries,
                                                  const d2q9_batch_item* __restrict__ items,
                                                  const int32_t* __restrict__ qb,
                                                  int32_t* __restrict__ counters, int parity,
@@ -98,16 +100,32 @@ __global__ __launch_bounds__(256) void k_batched(const d2q9_batch_entry* __restr
 template <int S>
 hipError_t launch(const d2q9_batch_entry* entries, const d2q9_batch_item* items,
                   const int32_t* qb, int32_t* counters, int parity, int tile_i,
-                  int grid, int policy, int chunk, int write_mom, hipStream_t s) {
-  hipLaunchKernelGGL((k_batched<S>), dim3(grid), dim3(64, tile_i), 0, s, entries,
-                     items, qb, counters, parity, policy, chunk, write_mom);
+                  int grid, int policy, int chunk, int waves, int write_mom,
+                  hipStream_t s) {
+  const dim3 g(grid), b(64, tile_i);
+  switch (waves) {
+    case 0:
+      hipLaunchKernelGGL((k_batched<S, 1>), g, b, 0, s, entries, items, qb, counters,
+                         parity, policy, chunk, write_mom);
+      break;
+    case 6:
+      hipLaunchKernelGGL((k_batched<S, 6>), g, b, 0, s, entries, items, qb, counters,
+                         parity, policy, chunk, write_mom);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_batched<S, 8>), g, b, 0, s, entries, items, qb, counters,
+                         parity, policy, chunk, write_mom);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
 #define D2Q9_BATCHED_DECLARE(SV)                                                        \
   extern template hipError_t launch<SV>(const d2q9_batch_entry*, const d2q9_batch_item*, \
                                         const int32_t*, int32_t*, int, int, int, int,   \
-                                        int, int, hipStream_t);
+                                        int, int, int, hipStream_t);
 D2Q9_BATCHED_DECLARE(D2Q9_STORE_F64)
 D2Q9_BATCHED_DECLARE(D2Q9_STORE_F32)
 #undef D2Q9_BATCHED_DECLARE

@@ -10,7 +10,8 @@ extern "C" int d2q9_batched_step(const d2q9_batch_entry* entries,
                                  const d2q9_batch_item* items,
                                  const int32_t* queue_bounds, int32_t* counters,
                                  int parity, int storage, int tile_i, int grid,
-                                 int policy, int chunk, int write_moments, void* stream) {
+                                 int policy, int chunk, int waves, int write_moments,
+                                 void* stream) {
   if (!entries || !items || !queue_bounds || !counters) {
     set_error("batched step: null buffer");
     return D2Q9_E_ARG;
@@ -20,8 +21,8 @@ extern "C" int d2q9_batched_step(const d2q9_batch_entry* entries,
     return D2Q9_E_ARG;
   }
   if (tile_i < 1 || tile_i > 4 || (parity != 0 && parity != 1) || policy < 0 || policy > 1 ||
-      chunk < 1) {
-    set_error("batched step: bad tile_i, parity, policy or chunk");
+      chunk < 1 || (waves != 0 && waves != 6 && waves != 8)) {
+    set_error("batched step: bad tile_i, parity, policy, chunk or waves");
     return D2Q9_E_ARG;
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -29,11 +30,11 @@ extern "C" int d2q9_batched_step(const d2q9_batch_entry* entries,
   switch (storage) {
     case D2Q9_STORE_F64:
       e = batched::launch<D2Q9_STORE_F64>(entries, items, queue_bounds, counters, parity,
-                                          tile_i, grid, policy, chunk, write_moments, s);
+                                          tile_i, grid, policy, chunk, waves, write_moments, s);
       break;
     case D2Q9_STORE_F32:
       e = batched::launch<D2Q9_STORE_F32>(entries, items, queue_bounds, counters, parity,
-                                          tile_i, grid, policy, chunk, write_moments, s);
+                                          tile_i, grid, policy, chunk, waves, write_moments, s);
       break;
     default:
       set_error("batched step: FP64 or FP32 storage only");

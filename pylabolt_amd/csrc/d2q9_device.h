@@ -115,6 +115,25 @@ __device__ __forceinline__ Consts<float> load_consts<float>(const d2q9_lattice& 
 //   dev == true:  the value is d_k = f_k - w_k; raw = d_k * scale_k
 // CPL: cells per lane, chosen so that one lane moves 8 bytes per direction.
 // ---------------------------------------------------------------------------
+// 32-bit integer hash ("lowbias32" finaliser, C. Wellons, 2018): xor-shift
+// and multiply rounds with full avalanche.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// Random bits for rounding value `b` (float bits) of direction k of cell
+// `salt`: a function of the data only, so a run is reproducible bit for
+// bit, while the rounding of a slowly changing field is decorrelated from
+// step to step.
+__device__ __forceinline__ uint32_t round_bits(uint32_t b, int k, uint32_t salt) {
+  return mix32(b ^ ((salt * 9u + (uint32_t)k) * 0x9e3779b9u));
+}
+
 template <int S> struct Codec;
 
 template <> struct Codec<D2Q9_STORE_F64> {
@@ -124,6 +143,7 @@ template <> struct Codec<D2Q9_STORE_F64> {
   static constexpr int CPL = 1;
   static __device__ __forceinline__ T dec(raw r, int, const Consts<T>&) { return r; }
   static __device__ __forceinline__ raw enc(T v, int, const Consts<T>&) { return v; }
+  static __device__ __forceinline__ raw encs(T v, int, uint32_t, const Consts<T>&) { return v; }
   static __device__ __forceinline__ double f64(raw r, int, const d2q9_lattice&) { return r; }
 };
 
@@ -134,6 +154,9 @@ template <> struct Codec<D2Q9_STORE_F32> {
   static constexpr int CPL = 2;
   static __device__ __forceinline__ T dec(raw r, int k, const Consts<T>& K) { return r * K.iscale[k]; }
   static __device__ __forceinline__ raw enc(T v, int k, const Consts<T>& K) { return v * K.scale[k]; }
+  static __device__ __forceinline__ raw encs(T v, int k, uint32_t, const Consts<T>& K) {
+    return enc(v, k, K);
+  }
   static __device__ __forceinline__ double f64(raw r, int k, const d2q9_lattice& L) {
     return L.w[k] + (double)r / L.dev_scale[k];
   }
@@ -150,6 +173,14 @@ template <> struct Codec<D2Q9_STORE_F16DEV> {
   static __device__ __forceinline__ raw enc(T v, int k, const Consts<T>& K) {
     return __float2half_rn(v * K.scale[k]);
   }
+  // Stochastic rounding to half: random bits added below the 10 kept
+  // mantissa bits, then truncation.  The truncated float is exact in half
+  // over half's normal range; below it the conversion rounds to nearest.
+  static __device__ __forceinline__ raw encs(T v, int k, uint32_t salt, const Consts<T>& K) {
+    const uint32_t b = __float_as_uint(v * K.scale[k]);
+    const uint32_t t = (b + (round_bits(b, k, salt) & 0x1fffu)) & ~0x1fffu;
+    return __float2half_rn(__uint_as_float(t));
+  }
   static __device__ __forceinline__ double f64(raw r, int k, const d2q9_lattice& L) {
     return L.w[k] + (double)__half2float(r) / L.dev_scale[k];
   }
@@ -165,6 +196,12 @@ template <> struct Codec<D2Q9_STORE_BF16DEV> {
   }
   static __device__ __forceinline__ raw enc(T v, int k, const Consts<T>& K) {
     return __float2bfloat16(v * K.scale[k]);
+  }
+  // Stochastic rounding to bfloat16 (the upper 16 bits of a float).
+  static __device__ __forceinline__ raw encs(T v, int k, uint32_t salt, const Consts<T>& K) {
+    const uint32_t b = __float_as_uint(v * K.scale[k]);
+    const uint32_t t = b + (round_bits(b, k, salt) & 0xffffu);
+    return __ushort_as_bfloat16((unsigned short)(t >> 16));
   }
   static __device__ __forceinline__ double f64(raw r, int k, const d2q9_lattice& L) {
     return L.w[k] + (double)__bfloat162float(r) / L.dev_scale[k];
@@ -426,9 +463,12 @@ __device__ __forceinline__ void apply_bc(
       T cu = (T)CX[o] * ex + (T)CY[o] * ey;
       const T rb = (T)e.scalar;
       if constexpr (Codec<S>::dev) {
-        // d = -d*[o] + (2 w rho_b (1 + ...) - 2 w)
+        // d = -d*[o] + (2 w rho_b (1 + ...) - 2 w); rho_b - 1 is formed in
+        // double so that the small imposed density difference keeps its
+        // digits in float
+        const T drb = (T)(e.scalar - 1.0);
         T temp = ((T)2 * K.w[o]) *
-                 ((rb - (T)1) + rb * (((T)0.5 * ics4) * cu * cu - ((T)0.5 * ics2) * u2));
+                 (drb + rb * (((T)0.5 * ics4) * cu * cu - ((T)0.5 * ics2) * u2));
         g[d] = -ld<S>(L, K, f, i, j, o) + temp;
       } else {
         T temp = (((T)2 * K.w[o]) * rb) *

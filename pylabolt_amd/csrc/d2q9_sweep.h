@@ -155,8 +155,9 @@ __device__ __forceinline__ void finish_cell(
     T drho, rho, ux, uy, fx, fy;
     moments<T, FORCING, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
     collide<T, MODEL, FORCING, C::dev>(g, K, drho, rho, ux, uy, fx, fy);
+    // 16-bit storage rounds stochastically (deterministic in the data)
 #pragma unroll
-    for (int k = 0; k < 9; ++k) out[k][q] = C::enc(g[k], k, K);
+    for (int k = 0; k < 9; ++k) out[k][q] = C::encs(g[k], k, (uint32_t)c, K);
     if (store_mom) {
       mdst[c] = rho;
       mdst[L.plane + c] = ux;

@@ -30,12 +30,15 @@ def test_items_cover_tiles_and_fix_cells_once(tile_i):
     items, bounds, owner = pack_queues(engines, per)
     assert bounds[0] == 0 and bounds[-1] == items.shape[0]
     assert np.all(np.diff(bounds) >= 0) and len(bounds) == N_XCD + 1
-    # every lattice sits in exactly one queue, with all of its items
-    seen = sorted(l for q in owner for l in q)
-    assert seen == list(range(len(engines)))
+    # the pieces partition every lattice's item list
+    for l in range(len(engines)):
+        spans = sorted((a, b) for q in owner for (m, a, b) in q if m == l)
+        assert spans[0][0] == 0 and spans[-1][1] == per[l].shape[0]
+        assert all(spans[k][1] == spans[k + 1][0]
+                   for k in range(len(spans) - 1))
     for q in range(N_XCD):
         lat = set(items[bounds[q]:bounds[q + 1], 0].tolist())
-        assert lat == set(owner[q])
+        assert lat == {m for (m, _, _) in owner[q]}
     for l, e in enumerate(engines):
         t = e.tables
         codes = items[items[:, 0] == l, 1]
@@ -49,18 +52,26 @@ def test_items_cover_tiles_and_fix_cells_once(tile_i):
                               np.arange(t.nti * t.ntj))
 
 
-def test_queues_balanced_by_cells():
-    sims = cases.batch_cases(n=64, seed=1, nx_range=(64, 512),
-                             ny_range=(64, 256))
-    engines = [SimpleNamespace(nx=int(s.mesh_dict["grid"][0]),
-                               ny=int(s.mesh_dict["grid"][1]), tables=None)
-               for s in sims]
-    per = [np.zeros((1, 2), dtype=np.int32) for _ in engines]
-    _, _, owner = pack_queues(engines, per)
-    load = [sum(engines[l].nx * engines[l].ny for l in q) for q in owner]
-    biggest = max(e.nx * e.ny for e in engines)
-    # LPT: the spread between queues is at most one lattice
-    assert max(load) - min(load) <= biggest
+@pytest.mark.parametrize("sizes", [
+    [5000],                                  # one large lattice
+    [40, 7, 300, 1200, 64, 64, 9, 500, 33],  # uneven
+    [100] * 64,                              # many equal
+])
+def test_queues_balanced(sizes):
+    engines = [SimpleNamespace(nx=1, ny=n, tables=None) for n in sizes]
+    per = [np.stack([np.full(n, l), np.arange(n)], axis=1).astype(np.int32)
+           for l, n in enumerate(sizes)]
+    items, bounds, owner = pack_queues(engines, per)
+    load = np.diff(bounds)
+    total = sum(sizes)
+    cap = -(-total // (N_XCD * 4))
+    # LPT over pieces of at most cap items: spread at most one piece
+    assert load.max() - load.min() <= cap
+    assert load.sum() == total
+    # every item exactly once
+    got = sorted(map(tuple, items.tolist()))
+    assert got == sorted((l, k) for l, n in enumerate(sizes)
+                         for k in range(n))
 
 
 def test_batch_workload_shape():

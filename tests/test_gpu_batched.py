@@ -46,20 +46,22 @@ def _assert_same(a, b, label):
 
 
 CONFIGS = [
-    (4, "static", None, 1),
-    (4, "dynamic", None, 1),
-    (4, "dynamic", 64, 4),
-    (2, "dynamic", 24, 3),
-    (1, "static", 8, 1),
-    (2, "static", 4096, 1),
+    (4, "static", None, 1, 0),
+    (4, "dynamic", None, 1, 0),
+    (4, "dynamic", 64, 4, 6),
+    (2, "dynamic", 24, 3, 8),
+    (1, "static", 8, 1, 6),
+    (2, "static", 4096, 1, 8),
 ]
 
 
-@pytest.mark.parametrize("tile_i,policy,grid,chunk", CONFIGS)
-def test_batched_fp32_bitwise_equals_per_lattice(tile_i, policy, grid, chunk):
+@pytest.mark.parametrize("tile_i,policy,grid,chunk,waves", CONFIGS)
+def test_batched_fp32_bitwise_equals_per_lattice(tile_i, policy, grid, chunk,
+                                                 waves):
     from pylabolt_amd.batch import BatchConfig, BatchedLattices
     cfg = BatchConfig.from_dict({"tile_i": tile_i, "policy": policy,
-                                 "grid": grid, "chunk": chunk})
+                                 "grid": grid, "chunk": chunk,
+                                 "waves": waves})
     sims = _mixed_sims()
     print(f"[batched] f32 {cfg} lattices "
           f"{[tuple(s.mesh_dict['grid']) for s in sims]}", flush=True)
@@ -148,12 +150,14 @@ def test_batched_rejects_bad_arguments():
                         config=BatchConfig(4, 1, None, 1))
     b._sync_entries()
     lib = nt.lib()
-    # grid not a multiple of 8, chunk 0, FP16 storage: refused on the host
-    for args in ((0, nt.STORE_F32, 4, 12, 1, 1), (0, nt.STORE_F32, 4, 16, 1, 0),
-                 (0, nt.STORE_F16DEV, 4, 16, 1, 1)):
-        p, st, ti, g, pol, ch = args
+    # grid not a multiple of 8, chunk 0, FP16 storage, waves 5: refused on
+    # the host
+    for args in ((0, nt.STORE_F32, 4, 12, 1, 1, 0), (0, nt.STORE_F32, 4, 16, 1, 0, 0),
+                 (0, nt.STORE_F16DEV, 4, 16, 1, 1, 0),
+                 (0, nt.STORE_F32, 4, 16, 1, 1, 5)):
+        p, st, ti, g, pol, ch, wv = args
         rc = lib.d2q9_batched_step(nt.ptr(b.entries), nt.ptr(b.items),
                                    nt.ptr(b.bounds), nt.ptr(b.counters),
-                                   p, st, ti, g, pol, ch, 0, b._sp)
+                                   p, st, ti, g, pol, ch, wv, 0, b._sp)
         assert rc != 0
     b.close()
