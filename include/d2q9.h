@@ -143,10 +143,12 @@ typedef struct d2q9_lattice {
   int32_t n_bulk, n_general;
   const int32_t* fix_cells;
   int32_t n_fix, pad_fix;
-  /* Special-cell bits of the bulk tiles: for the tile at position p of the
+  /* Masked-cell bits of the bulk tiles: for the tile at position p of the
    * bulk list and its row r, words [(p * tile_i + r) * cpl, + cpl) hold one
    * bit per cell of the block (bit b of word w: cell 1 + tj*tile_j + 64w + b).
-   * The bulk sweep does not store those cells; the fix-up pass does. */
+   * The bulk sweep does not store those cells: the special ones are
+   * recomputed by the fix-up pass, the ones past the lattice's last column
+   * (partial tiles) are never written. */
   const uint64_t* bulk_masks;
   const int32_t* bc_slot;       /* [plane], read where D2Q9_CELL_BC */
   const d2q9_bc_node* bc_nodes;

@@ -180,9 +180,8 @@ class BatchedLattices:
         if cfg.grid is None:
             cfg.grid = self.resident_grid(cfg.tile_i, cfg.waves)
         cfg.grid = max(N_XCD, int(cfg.grid) // N_XCD * N_XCD)
-        if self.config is not None and cfg.tile_i != self.config.tile_i:
-            self.stream.synchronize()
         if self.config is None or cfg.tile_i != self.config.tile_i:
+            self.stream.synchronize()
             for e in self.engines:
                 e.set_tile_rows(cfg.tile_i)
             items, bounds, self.owner = pack_queues(

@@ -187,11 +187,12 @@ __device__ __forceinline__ void store_block(
   }
 }
 
-// Bulk tile at position p of the tile list (host-classified: every cell
-// fluid and inside the lattice).  Every cell is swept as plain interior
-// fluid and no flag is read; the cells marked in bulk_masks (solid
-// neighbour, boundary rule, stored moment, periodic edge) are not stored
-// here -- the fix-up pass writes them.
+// Bulk tile at position p of the tile list (host-classified: no solid
+// cell).  Every cell is swept as plain interior fluid and no flag is read;
+// the cells marked in bulk_masks are not stored here: the special ones
+// (solid neighbour, boundary rule, stored moment, periodic edge) are
+// written by the fix-up pass, the ones past the lattice's last column are
+// never written.
 template <int S, int MODEL, int FORCING, int PHASE>
 __device__ __forceinline__ void bulk_tile(
     const d2q9_lattice& L, const typename Codec<S>::raw* __restrict__ fsrc,
@@ -206,7 +207,7 @@ __device__ __forceinline__ void bulk_tile(
   const int lane = (int)threadIdx.x;
   const int i = 1 + ti * (int)blockDim.y + (int)threadIdx.y;
   const int j0 = 1 + tj * B + lane * CPL;
-  if (i > L.nx) return;  // whole wave (never for a bulk tile)
+  if (i > L.nx) return;  // whole wave (last tile row of a partial tile)
   const Consts<T> K = load_consts<T>(L);
   // special bits of this lane's cells
   const uint64_t* mrow = L.bulk_masks + ((int64_t)p * blockDim.y + threadIdx.y) * CPL;

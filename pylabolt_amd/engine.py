@@ -150,11 +150,14 @@ class CaseTables:
     def set_tile(self, tile_j, tile_i):
         """Classify tiles of tile_j x tile_i interior cells.
 
-        Bulk tiles (class 0) hold only fluid cells inside the lattice; the
-        bulk sweep treats all their cells as plain interior fluid, and the
-        cells of bulk tiles that are not -- solid neighbour, boundary rule,
-        stored moment, periodic edge -- go to the fix-up list.  General
-        tiles (class 1) hold a solid cell or reach past the lattice.
+        Bulk tiles (class 0) hold no solid cell: the bulk sweep treats all
+        their cells as plain interior fluid and stores only the cells their
+        mask leaves set.  Masked out are the special cells -- solid
+        neighbour, boundary rule, stored moment, periodic edge -- which go
+        to the fix-up list, and the cells past the lattice's last column of
+        a partial tile, which are never stored (the ghost column keeps its
+        initial zero; rows past the last row are not swept).  General tiles
+        (class 1) hold a solid cell.
         """
         if tile_j % 64 or not 1 <= tile_i <= 4:
             raise ValueError("tile_j must be 64 * cpl and tile_i 1..4")
@@ -172,7 +175,9 @@ class CaseTables:
         special &= fluid
         ntj = -(-ny // self.tile_j)
         nti = -(-nx // self.tile_i)
-        padded = np.zeros((nti * self.tile_i, ntj * self.tile_j), dtype=bool)
+        # cells past the lattice count as fluid here: they do not stop a
+        # tile from being swept as bulk
+        padded = np.ones((nti * self.tile_i, ntj * self.tile_j), dtype=bool)
         padded[:nx, :ny] = fluid
         tiles = padded.reshape(nti, self.tile_i, ntj, self.tile_j).all(axis=(1, 3))
         self.tile_class = np.where(tiles, 0, 1).astype(np.uint8).ravel()
@@ -185,8 +190,9 @@ class CaseTables:
         # fix-up cells: special cells lying in bulk tiles
         in_bulk = np.repeat(np.repeat(tiles, self.tile_i, axis=0),
                             self.tile_j, axis=1)[:nx, :ny]
-        # special-cell bits per (bulk tile, row, 64-cell word)
-        spad = np.zeros((nti * self.tile_i, ntj * self.tile_j), dtype=bool)
+        # masked bits per (bulk tile, row, 64-cell word): special cells
+        # and cells past the lattice
+        spad = np.ones((nti * self.tile_i, ntj * self.tile_j), dtype=bool)
         spad[:nx, :ny] = special
         words = self.tile_j // 64
         bits = spad.reshape(nti, self.tile_i, ntj, words, 64)

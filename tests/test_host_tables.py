@@ -26,6 +26,8 @@ def _tables(name, B):
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("B", [64, 128, 256])
 def test_bulk_masks_match_fix_cells(name, B):
+    """Masked bits inside the lattice are exactly the fix-up cells; every
+    cell past the lattice's last column or row is masked."""
     st, t, nx, ny = _tables(name, B)
     pitch, _ = E.geometry(nx, ny)
     fix = set(int(c) for c in t.fix_cells[:t.n_fix])
@@ -39,9 +41,12 @@ def test_bulk_masks_match_fix_cells(name, B):
             for w in range(B // 64):
                 word = int(masks[p, r, w])
                 for b in range(64):
-                    if (word >> b) & 1:
-                        i = 1 + ti * 4 + r
-                        j = 1 + tj * B + 64 * w + b
+                    i = 1 + ti * 4 + r
+                    j = 1 + tj * B + 64 * w + b
+                    inside = i <= nx and j <= ny
+                    if not inside:
+                        assert (word >> b) & 1, (p, r, w, b)
+                    elif (word >> b) & 1:
                         seen.add(E.ORIGIN + i * pitch + j)
     assert seen == fix
 
@@ -52,11 +57,17 @@ def test_tile_classes_partition(name):
     assert t.n_bulk + t.n_general == t.ntj * t.nti
     assert sorted(t.tile_list.tolist()) == list(range(t.ntj * t.nti))
     fluid = t.flags_pad[1:-1, 1:-1] & 1
+    solid = (t.flags_pad[1:-1, 1:-1] & 2) != 0
     for p in range(t.n_bulk):
         tid = int(t.tile_list[p])
         tj, ti = tid % t.ntj, tid // t.ntj
+        # the part of the tile inside the lattice is all fluid
         blk = fluid[ti * 4:ti * 4 + 4, tj * 64:tj * 64 + 64]
-        assert blk.shape == (4, 64) and blk.all()
+        assert blk.size and blk.all()
+    for p in range(t.n_bulk, t.n_bulk + t.n_general):
+        tid = int(t.tile_list[p])
+        tj, ti = tid % t.ntj, tid // t.ntj
+        assert solid[ti * 4:ti * 4 + 4, tj * 64:tj * 64 + 64].any()
 
 
 def test_cavity_corner_rules_last_element_wins():
