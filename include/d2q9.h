@@ -29,6 +29,10 @@
  *                        then moments + collision of step n+1)
  *   d2q9_moments      <- compute_fields_kernels.py:5-65 alone
  *   d2q9_mass_fp64    <- no reference kernel; FP64 mass check
+ *   d2q9_momentum_exchange <- parallel/cpu/force_torque_kernels.py:11-141
+ *                        (obstacle force/torque and wall force), bound by
+ *                        base/obstacle_operator.py:75-113 and
+ *                        base/boundary_operator.py:204-294
  *   d2q9_residues     <- parallel/cpu/compute_residues_kernels.py:6-73,
  *                        utils/residues.py:171-317
  *   d2q9_graph_*      <- solvers/fluidLB.py:270-287 (CUDA graph of a step)
@@ -209,6 +213,41 @@ int d2q9_graph_begin(void* stream);
 int d2q9_graph_end(void* stream, void** graph_exec);
 int d2q9_graph_launch(void* graph_exec, void* stream);
 int d2q9_graph_destroy(void* graph_exec);
+
+/*
+ * Momentum exchange on obstacle and wall links.  A force node is a fluid
+ * cell and a 9-bit mask of directions k whose link it sums:
+ *   F += f*[k] c_k - f_new[inv k] c_{inv k}     (FP64 sums)
+ * with f* the post-collision populations of the current step (f_pop) and
+ * f_new their streamed values after the boundary rules (read from
+ * f_pop_new when it is given, else recomputed for the node from f_pop and
+ * mom).  A group with torque != 0 also sums r_x F_y - r_y F_x over its
+ * links, r = (i - 1, j - 1) - ref with the minimum image across periodic
+ * sides.  Nodes are sorted by group; each group owns n_blocks whole blocks
+ * of 256 nodes starting at block first_block (padding nodes have mask 0),
+ * n_nodes is a multiple of 256, work holds 3 doubles per block and out 3
+ * per group (fx, fy, torque).  The sums are in a fixed order: a run is
+ * reproducible bit for bit.
+ */
+typedef struct d2q9_force_node {
+  int64_t cell;   /* origin + i * pitch + j */
+  int32_t group;
+  uint32_t mask;  /* bit k: sum the link of direction k */
+} d2q9_force_node;
+
+typedef struct d2q9_force_group {
+  double ref[2];
+  int32_t torque;
+  int32_t first_block;
+  int32_t n_blocks;
+  int32_t pad;
+} d2q9_force_group;
+
+int d2q9_momentum_exchange(const d2q9_lattice* L, const void* f_pop,
+                           const void* f_pop_new, const void* mom,
+                           const d2q9_force_node* nodes, int64_t n_nodes,
+                           const d2q9_force_group* groups, int n_groups,
+                           double* work, double* out, void* stream);
 
 /*
  * Batched sweep: many lattices advance one fused step in one persistent

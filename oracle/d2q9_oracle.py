@@ -415,3 +415,88 @@ def case_from_state(state, model, forcing, omega, gravity,
                       state.boundary.y_periodic, omega, model=model,
                       forcing=forcing, gravity=gravity,
                       force_source=force_source, force_field=force_field)
+
+
+# ---------------------------------------------------------------------------
+# Momentum exchange (forces on obstacles and walls)
+# ---------------------------------------------------------------------------
+
+def fluid_boundary_nodes(shape, solid, ghost, solid_id):
+    """compute_obstacle_boundary, fluid branch
+    (parallel/cpu/obstacle_kernels.py:338-348): a non-ghost fluid node with
+    a solid neighbour at ind + c_k is a fluid boundary node and takes the
+    solid_id of the first such neighbour.  Returns (fluid_boundary, node
+    solid_id) arrays over the padded lattice."""
+    nyp = shape[1]
+    fb = np.zeros(shape[0] * nyp, dtype=bool)
+    sid = np.asarray(solid_id).copy()
+    for ind in range(shape[0] * nyp):
+        if ghost[ind] or solid[ind]:
+            continue
+        x, y = divmod(ind, nyp)
+        for k in range(9):
+            nb = (x + CX[k]) * nyp + (y + CY[k])
+            if solid[nb]:
+                fb[ind] = True
+                sid[ind] = sid[nb]
+                break
+    return fb, sid
+
+
+def obstacle_force_torque(case, solid_id, fluid_boundary, ref_point,
+                          current_id, pop=None, pop_new=None):
+    """compute_force_torque_single_phase
+    (parallel/cpu/force_torque_kernels.py:11-95), sequential, with the
+    single-rank offset 0: over fluid boundary nodes of obstacle
+    ``current_id``, for every direction k whose neighbour is solid,
+    value = pop[k] c_k - pop_new[inv k] c_{inv k}; torque about
+    ``ref_point`` with the minimum image across periodic sides."""
+    pop = case.pop if pop is None else pop
+    pop_new = case.pop_new if pop_new is None else pop_new
+    nxp, nyp = case.shape
+    Nx, Ny = nxp - 2, nyp - 2
+    fx = fy = tq = 0.0
+    for ind in range(case.size):
+        if case.ghost[ind] or not fluid_boundary[ind]:
+            continue
+        if solid_id[ind] != current_id:
+            continue
+        x, y = divmod(ind, nyp)
+        rx = (x - 1) - ref_point[0]
+        ry = (y - 1) - ref_point[1]
+        rxm, rym = rx, ry
+        if case.x_periodic:
+            if abs(rx + Nx) < abs(rxm):
+                rxm = rx + Nx
+            if abs(rx - Nx) < abs(rxm):
+                rxm = rx - Nx
+        if case.y_periodic:
+            if abs(ry + Ny) < abs(rym):
+                rym = ry + Ny
+            if abs(ry - Ny) < abs(rym):
+                rym = ry - Ny
+        for k in range(9):
+            nb = (x + CX[k]) * nyp + (y + CY[k])
+            if case.solid[nb]:
+                kb = INV[k]
+                vx = pop[ind, k] * CX[k] - pop_new[ind, kb] * CX[kb]
+                vy = pop[ind, k] * CY[k] - pop_new[ind, kb] * CY[kb]
+                fx += vx
+                fy += vy
+                tq += rxm * vy - rym * vx
+    return np.array([fx, fy, tq])
+
+
+def boundary_force(case, element, pop=None, pop_new=None):
+    """compute_boundary_force_single_phase
+    (parallel/cpu/force_torque_kernels.py:98-141) for one element."""
+    pop = case.pop if pop is None else pop
+    pop_new = case.pop_new if pop_new is None else pop_new
+    fx = fy = 0.0
+    for ind in element["nodes"]:
+        if case.solid[ind]:
+            continue
+        for ko, ki in zip(element["out"], element["inv"]):
+            fx += pop[ind, ko] * CX[ko] - pop_new[ind, ki] * CX[ki]
+            fy += pop[ind, ko] * CY[ko] - pop_new[ind, ki] * CY[ki]
+    return np.array([fx, fy])

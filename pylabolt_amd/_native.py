@@ -74,6 +74,18 @@ class Lattice(ctypes.Structure):
     ]
 
 
+# numpy mirrors of d2q9_force_node (16 B) and d2q9_force_group (32 B)
+import numpy as _np  # noqa: E402
+
+FORCE_NODE_DTYPE = _np.dtype([("cell", "<i8"), ("group", "<i4"),
+                              ("mask", "<u4")])
+FORCE_GROUP_DTYPE = _np.dtype([("ref", "<f8", (2,)), ("torque", "<i4"),
+                               ("first_block", "<i4"), ("n_blocks", "<i4"),
+                               ("pad", "<i4")])
+FORCE_BLOCK = 256
+assert FORCE_NODE_DTYPE.itemsize == 16 and FORCE_GROUP_DTYPE.itemsize == 32
+
+
 class BatchEntry(ctypes.Structure):
     """Mirror of ``d2q9_batch_entry``."""
     _fields_ = [("L", Lattice), ("f", ctypes.c_void_p * 2),
@@ -112,6 +124,9 @@ SIGNATURES = {
     "d2q9_adjoint_init_grad": (ctypes.c_int, [_LP, _P, _P, _P, _P]),
     "d2q9_batched_step": (ctypes.c_int, [_P, _P, _P, _P] + [ctypes.c_int] * 8
                           + [_P]),
+    "d2q9_momentum_exchange": (ctypes.c_int, [_LP, _P, _P, _P, _P,
+                                              ctypes.c_int64, _P, ctypes.c_int,
+                                              _P, _P, _P]),
 }
 
 # fields whose offsets d2q9_abi_layout reports, in its order

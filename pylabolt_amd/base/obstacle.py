@@ -43,6 +43,9 @@ class Circle:
                 "center must be list for obstacle type circle: " + name
             )
         self.center = np.array(center, dtype=control.precision)
+        # torque reference point of a circle: its centre (obstacle.py:450)
+        self.ref_point = self.center
+        self.inclination_angle = control.precision(0)
         if "density" not in user_dict:
             raise ValueError("density missing in obstacle: " + name)
         solid_density = user_dict["density"]
@@ -127,6 +130,30 @@ class Obstacle:
                     "compute_force_torque must be bool:" +
                     " True/False (default: False)"
                 )
+        # options of obstacle.py:147-190
+        if "ref_point_torque" in options:
+            if self.compute_force_torque is False:
+                print_log("WARNING! ref_point_torque ignored in "
+                          "obstacle_dict options as compute_force_torque "
+                          "is False", domain.mpi_rank, verbose)
+            else:
+                ref_point_torque = options["ref_point_torque"]
+                if not isinstance(ref_point_torque, list):
+                    raise ValueError("ref_point_torque must be list: (x, y)")
+                self.ref_point_torque = np.array(ref_point_torque,
+                                                 dtype=control.precision)
+        if "write_obstacle_data" in options:
+            self.write_obstacle_data = True
+            temp_dict = options["write_obstacle_data"]
+            if "interval" not in temp_dict:
+                raise ValueError("interval missing obstacle_dict options!")
+            self.write_interval = temp_dict["interval"]
+            if not isinstance(self.write_interval, (int, type(None))):
+                raise ValueError(
+                    "interval must be int or None in obstacle_dict options")
+            if isinstance(self.write_interval, int) and self.write_interval <= 0:
+                raise ValueError("if interval is int then it must be > 0"
+                                 " in obstacle_dict options")
         keys = [k for k in self.obstacle_dict.keys() if k != "options"]
         for obstacle_id, key in enumerate(keys):
             user_dict = self.obstacle_dict[key]

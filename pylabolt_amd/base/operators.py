@@ -12,8 +12,65 @@ against the reference runs unchanged; each verb maps onto the engine:
   (streaming_kernels.py:5-47, fluid_boundary_kernels.py:5-152);
 * ``halo_exchange``: nothing to do on one GPU, the periodic wrap is folded
   into the streaming index (MPI_kernels.py:34-99);
-* ``move_obstacles`` / ``compute_force_torque``: static bodies only.
+* ``move_obstacles``: static bodies only;
+* ``compute_force`` / ``compute_force_torque``: momentum exchange on the
+  wall and obstacle links (force_torque_kernels.py:11-141), one FP64
+  reduction over the boundary nodes (d2q9_momentum_exchange).
 """
+import numpy as np
+
+from pylabolt_amd.base.lattice import CX, CY
+
+
+def obstacle_boundary_nodes(state):
+    """Fluid boundary nodes of the obstacles, their owning obstacle and the
+    direction mask of their solid links.
+
+    Reference: compute_obstacle_boundary (parallel/cpu/obstacle_kernels.py,
+    the ``elif not solid[ind]`` branch): a non-ghost fluid node with a solid
+    neighbour at (i + c_k) is a fluid boundary node and takes the solid_id
+    of the first such neighbour in direction order; the force kernel
+    (force_torque_kernels.py:45-90) sums every direction whose neighbour is
+    solid.  Returns (nodes, masks, ids), nodes as padded indices."""
+    nxp, nyp = (int(v) for v in state.domain.shape)
+    f = state.fields
+    solid = np.asarray(f.solid, bool).reshape(nxp, nyp)
+    ghost = np.asarray(f.ghost_node, bool).reshape(nxp, nyp)
+    sid = np.asarray(f.solid_id).reshape(nxp, nyp)
+    cand = ~solid & ~ghost
+    mask = np.zeros((nxp, nyp), dtype=np.uint32)
+    first = np.full((nxp, nyp), -1, dtype=np.int64)
+    ii, jj = np.nonzero(cand)
+    for k in range(9):
+        ni, nj = ii + CX[k], jj + CY[k]
+        hit = solid[ni, nj]
+        mask[ii[hit], jj[hit]] |= np.uint32(1 << k)
+        take = hit & (first[ii, jj] < 0)
+        first[ii[take], jj[take]] = sid[ni[take], nj[take]]
+    sel = mask[ii, jj] != 0
+    nodes = ii[sel] * nyp + jj[sel]
+    return nodes, mask[ii[sel], jj[sel]], first[ii[sel], jj[sel]]
+
+
+def wall_boundary_groups(state):
+    """One group per wall element: its non-solid nodes and the out_list
+    directions (compute_boundary_force_single_phase,
+    force_torque_kernels.py:98-141)."""
+    f = state.fields
+    solid = np.asarray(f.solid, bool)
+    groups = []
+    for e in state.boundary.boundary_elements:
+        if not e.wall:
+            continue
+        nodes = np.asarray(e.boundary_nodes, dtype=np.int64)
+        nodes = nodes[~solid[nodes]]
+        m = 0
+        for k in e.out_list:
+            m |= 1 << int(k)
+        groups.append({"element": e, "nodes": nodes,
+                       "masks": np.full(nodes.size, m, dtype=np.uint32),
+                       "torque": False})
+    return groups
 
 
 class ComputeFieldsOperator:
@@ -69,7 +126,25 @@ class BoundaryOperator:
         return None
 
     def compute_force(self, state, backend, mpi_operator, time_step=0):
-        return None
+        """Force on every wall element by momentum exchange, at the steps
+        the reference computes it (boundary_operator.py:204-240): when
+        ``compute_force`` is on and ``time_step`` is a multiple of the
+        boundary write interval.  Sets ``element.force``."""
+        b = state.boundary
+        if not b.compute_force or b.write_interval is None:
+            return
+        if time_step % b.write_interval != 0:
+            return
+        eng = backend.engine
+        if getattr(self, "_groups", None) is None:
+            self._groups = wall_boundary_groups(state)
+            self._handle = eng.set_force_groups(self._groups)
+        if not self._groups:
+            return
+        out = eng.momentum_exchange(self._handle)
+        for g, row in zip(self._groups, out):
+            g["element"].force[0] = row[0]
+            g["element"].force[1] = row[1]
 
 
 class MPIOperator:
@@ -108,4 +183,25 @@ class ObstacleOperator:
         return None
 
     def compute_force_torque(self, state, backend, mpi_operator):
-        return None
+        """Force and torque on every obstacle by momentum exchange
+        (obstacle_operator.py:75-113, force_torque_kernels.py:11-95); the
+        torque is taken about the obstacle's reference point (a circle's
+        centre).  Sets ``obstacle.force`` and ``obstacle.torque``."""
+        obs = state.obstacle
+        if not obs.compute_force_torque or obs.no_of_obstacles == 0:
+            return
+        eng = backend.engine
+        if getattr(self, "_handle", None) is None:
+            nodes, masks, ids = obstacle_boundary_nodes(state)
+            groups = []
+            for o in obs.obstacles:
+                sel = ids == o.id
+                groups.append({"nodes": nodes[sel], "masks": masks[sel],
+                               "ref": tuple(float(v) for v in o.ref_point),
+                               "torque": True})
+            self._handle = eng.set_force_groups(groups)
+        out = eng.momentum_exchange(self._handle)
+        for o, row in zip(obs.obstacles, out):
+            o.force[0] = row[0]
+            o.force[1] = row[1]
+            o.torque = row[2]

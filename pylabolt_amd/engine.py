@@ -213,6 +213,38 @@ class CaseTables:
         return self.tile_class
 
 
+def force_tables(nx, ny, groups):
+    """Device-ready node and group records for d2q9_momentum_exchange.
+
+    ``groups``: list of dicts with ``nodes`` (reference padded indices
+    ``i * (ny + 2) + j``), ``masks`` (direction bits per node), ``ref``
+    (x, y) and ``torque`` (bool).  Each group's nodes fill whole blocks of
+    256 records; padding records have mask 0."""
+    pitch, _ = geometry(nx, ny)
+    nyp = ny + 2
+    recs, grecs = [], np.zeros(len(groups), dtype=nt.FORCE_GROUP_DTYPE)
+    block = 0
+    for g, spec in enumerate(groups):
+        nodes = np.asarray(spec["nodes"], dtype=np.int64)
+        masks = np.asarray(spec["masks"], dtype=np.uint32)
+        n = int(nodes.size)
+        nb = -(-n // nt.FORCE_BLOCK)
+        r = np.zeros(nb * nt.FORCE_BLOCK, dtype=nt.FORCE_NODE_DTYPE)
+        i, j = nodes // nyp, nodes % nyp
+        r["cell"][:n] = ORIGIN + i * pitch + j
+        r["group"][:] = g
+        r["mask"][:n] = masks
+        recs.append(r)
+        grecs[g]["ref"] = spec.get("ref", (0.0, 0.0))
+        grecs[g]["torque"] = int(bool(spec.get("torque", False)))
+        grecs[g]["first_block"] = block
+        grecs[g]["n_blocks"] = nb
+        block += nb
+    nodes = (np.concatenate(recs) if recs else
+             np.zeros(0, dtype=nt.FORCE_NODE_DTYPE))
+    return nodes, grecs
+
+
 def elements_from_boundary(boundary):
     """Ordered element dicts from a ``Boundary`` (reference attribute
     names, base/boundary.py:7-84)."""
@@ -539,6 +571,43 @@ class D2Q9Engine:
             nt.ptr(self.red_out), self._stream_ptr), "d2q9_mass_fp64")
         self.synchronize()
         return float(self.red_out[0].item())
+
+    def set_force_groups(self, groups):
+        """Upload the momentum-exchange node and group tables (see
+        ``force_tables``); returns a handle for ``momentum_exchange``."""
+        nodes, grecs = force_tables(self.nx, self.ny, groups)
+        nblocks = max(1, nodes.size // nt.FORCE_BLOCK)
+        with torch.cuda.stream(self.stream):
+            h = {
+                "nodes": torch.from_numpy(nodes.view(np.uint8).copy()).to(
+                    self.device),
+                "groups": torch.from_numpy(grecs.view(np.uint8).copy()).to(
+                    self.device),
+                "n_nodes": int(nodes.size), "n_groups": int(grecs.size),
+                "work": torch.zeros(3 * nblocks, dtype=torch.float64,
+                                    device=self.device),
+                "out": torch.zeros(3 * max(1, grecs.size),
+                                   dtype=torch.float64, device=self.device),
+            }
+        return h
+
+    def momentum_exchange(self, handle):
+        """(n_groups, 3) FP64 array of force x, force y and torque on each
+        group's links for the current step: f* is the current post-
+        collision state, f_new its streamed and boundary-treated values
+        (read when in memory, else recomputed for the nodes alone)."""
+        fpop = self.f[self.i_pop]
+        fnew = (nt.ptr(self.f[self.i_pop_new]) if self.i_pop_new is not None
+                else None)
+        nt.check(self.lib.d2q9_momentum_exchange(
+            self._Lp, nt.ptr(fpop), fnew, nt.ptr(self.mom[self.i_mom]),
+            nt.ptr(handle["nodes"]), handle["n_nodes"],
+            nt.ptr(handle["groups"]), handle["n_groups"],
+            nt.ptr(handle["work"]), nt.ptr(handle["out"]), self._stream_ptr),
+            "d2q9_momentum_exchange")
+        self.synchronize()
+        return handle["out"][:3 * handle["n_groups"]].cpu().numpy().reshape(
+            -1, 3)
 
     def residue_sums(self, mom_old):
         """Six FP64 residue sums of the current moments against

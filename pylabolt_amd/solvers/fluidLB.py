@@ -132,8 +132,11 @@ class Solver:
         self.execute_single_time_step = self.single_time_step
 
     def single_time_step(self, verbose=True):
-        """One reference time step (fluidLB.py:206-253)."""
+        """One reference time step (fluidLB.py:206-253), obstacle forces
+        included when they are asked for."""
         self.engine.step(write_moments=True)
+        self.obstacle_operator.compute_force_torque(self.state, self.backend,
+                                                    self.mpi_operator)
 
     def advance(self, steps, write_moments=True):
         """``steps`` time steps, replaying HIP graphs between the ends."""
@@ -147,8 +150,13 @@ class Solver:
         st = self.state
         self.collision_operator.initialize_pop(st, self.backend)
         self.residue_operator.set_backend(st, self.backend)
+        self.boundary_operator.compute_force(st, self.backend,
+                                             self.mpi_operator, time_step=0)
+        self.obstacle_operator.compute_force_torque(st, self.backend,
+                                                    self.mpi_operator)
         self.io_operator.write_fields(st, self.backend, st.control.start_time,
                                       solver=self)
+        self.io_operator.write_histories(st, time_step=0)
         t0 = time.perf_counter()
         t = st.control.start_time
         end = st.control.end_time
@@ -156,23 +164,44 @@ class Solver:
             nxt = self._next_event(t, end)
             self.advance(nxt - t, write_moments=True)
             t = nxt
+            self.boundary_operator.compute_force(st, self.backend,
+                                                 self.mpi_operator, t)
+            if self._obstacle_event(t):
+                self.obstacle_operator.compute_force_torque(
+                    st, self.backend, self.mpi_operator)
             self.residue_operator.compute_residues(st, self.backend,
                                                    self.mpi_operator, t)
             self.logger.log_data(
                 st, t, res_density=self.residue_operator.residues["res_density"],
                 res_velocity=self.residue_operator.residues["res_velocity"])
             self.io_operator.write_fields(st, self.backend, t, solver=self)
+            self.io_operator.write_histories(st, t)
         self.engine.synchronize()
         self.run_time = time.perf_counter() - t0
         print_log(f"{'Simulation complete, run time':<30}: "
                   f"{str(self.run_time) + ' s':<30}", 0, self.verbose)
 
+    def _event_intervals(self):
+        st = self.state
+        ivs = [st.control.std_out_interval, st.control.save_interval]
+        if st.boundary.compute_force:
+            ivs.append(st.boundary.write_interval)
+        if st.obstacle.compute_force_torque and st.obstacle.write_obstacle_data:
+            ivs.append(st.obstacle.write_interval)
+        return [iv for iv in ivs if iv]
+
+    def _obstacle_event(self, t):
+        """Obstacle forces only feed the histories for static bodies: they
+        are evaluated at the obstacle write steps (the reference evaluates
+        them every step, fluidLB.py:249-253)."""
+        obs = self.state.obstacle
+        return (obs.compute_force_torque and obs.write_obstacle_data and
+                obs.write_interval and t % obs.write_interval == 0)
+
     def _next_event(self, t, end):
         nxt = end
-        for iv in (self.state.control.std_out_interval,
-                   self.state.control.save_interval):
-            if iv:
-                nxt = min(nxt, (t // iv + 1) * iv)
+        for iv in self._event_intervals():
+            nxt = min(nxt, (t // iv + 1) * iv)
         return nxt
 
     def verify_kernel_signatures(self, verbose=True):

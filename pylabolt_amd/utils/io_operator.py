@@ -27,7 +27,9 @@ class InputOutputOperator:
             self.fields_save_metadata[name] = {"components": comps,
                                                "dtype": str(field.dtype)}
         self.field_save_path = os.path.join(root, "output", "fields")
+        self.history_save_path = os.path.join(root, "output", "histories")
         self._dumped = False
+        self.setup_write_histories(state)
 
     def dump_metadata(self, state):
         meta = {
@@ -80,5 +82,61 @@ class InputOutputOperator:
         np.savez(os.path.join(self.field_save_path,
                               "t_" + str(time_step) + ".npz"), **out)
 
+    def setup_write_histories(self, state):
+        """History files of the obstacles and wall elements, with the
+        reference's headers (io_operator.py:236-296)."""
+        obs, bnd = state.obstacle, state.boundary
+        if not (obs.write_obstacle_data or bnd.write_boundary_data):
+            return
+        os.makedirs(self.history_save_path, exist_ok=True)
+        if obs.write_obstacle_data:
+            for o in obs.obstacles:
+                with open(os.path.join(self.history_save_path,
+                                       o.name + ".dat"), "w") as fh:
+                    fh.write(f"{'#':5} {'PyLaBolt obstacle history'}\n"
+                             f"{'#':5} {'ID':8}: {o.id}\n"
+                             f"{'#':5} {'Name':8}: {o.name}\n"
+                             f"{'#':5} {'Type':8}: {o.type}\n"
+                             f"{'#':5} {'Columns':8}:\n")
+                    fh.write(f"{'#':5}{'time':<21}{'pos_x':<30}{'pos_y':<30}"
+                             f"{'alpha':<30}{'vel_x':<30}{'vel_y':<30}"
+                             f"{'omega':<30}{'force_x':<30}{'force_y':<30}"
+                             f"{'torque':<30}\n")
+        if bnd.write_boundary_data:
+            for e in bnd.boundary_elements:
+                if not e.wall:
+                    continue
+                with open(os.path.join(self.history_save_path,
+                                       e.name + ".dat"), "w") as fh:
+                    fh.write(f"{'#':5} {'PyLaBolt boundary history'}\n"
+                             f"{'#':5} {'Name':8}: {e.name}\n"
+                             f"{'#':5} {'Columns':8}:\n")
+                    fh.write(f"{'#':5}{'time':<21}{'force_x':<30}"
+                             f"{'force_y':<30}\n")
+
     def write_histories(self, state, time_step):
-        return None
+        """Append one line per obstacle / wall element at the write
+        interval (io_operator.py:297-357)."""
+        obs, bnd = state.obstacle, state.boundary
+        if (obs.write_obstacle_data and obs.write_interval is not None
+                and time_step % obs.write_interval == 0):
+            for o in obs.obstacles:
+                with open(os.path.join(self.history_save_path,
+                                       o.name + ".dat"), "a") as fh:
+                    fh.write(f"{time_step:<24}"
+                             f"{o.center[0]:<30.16e}{o.center[1]:<30.16e}"
+                             f"{o.inclination_angle:<30.16e}"
+                             f"{o.linear_velocity[0]:<30.16e}"
+                             f"{o.linear_velocity[1]:<30.16e}"
+                             f"{o.angular_velocity:<30.16e}"
+                             f"{o.force[0]:<30.16e}{o.force[1]:<30.16e}"
+                             f"{o.torque:<30.16e}\n")
+        if (bnd.write_boundary_data and bnd.write_interval is not None
+                and time_step % bnd.write_interval == 0):
+            for e in bnd.boundary_elements:
+                if not e.wall:
+                    continue
+                with open(os.path.join(self.history_save_path,
+                                       e.name + ".dat"), "a") as fh:
+                    fh.write(f"{time_step:<24}{e.force[0]:<30.16e}"
+                             f"{e.force[1]:<30.16e}\n")
