@@ -8,6 +8,7 @@ fallback: if the library is missing, or no GPU is visible, ``lib()`` raises.
 import ctypes
 import os
 
+import numpy as np
 import torch  # noqa: F401  (must precede loading the HIP library)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -75,11 +76,9 @@ class Lattice(ctypes.Structure):
 
 
 # numpy mirrors of d2q9_force_node (16 B) and d2q9_force_group (32 B)
-import numpy as _np  # noqa: E402
-
-FORCE_NODE_DTYPE = _np.dtype([("cell", "<i8"), ("group", "<i4"),
+FORCE_NODE_DTYPE = np.dtype([("cell", "<i8"), ("group", "<i4"),
                               ("mask", "<u4")])
-FORCE_GROUP_DTYPE = _np.dtype([("ref", "<f8", (2,)), ("torque", "<i4"),
+FORCE_GROUP_DTYPE = np.dtype([("ref", "<f8", (2,)), ("torque", "<i4"),
                                ("first_block", "<i4"), ("n_blocks", "<i4"),
                                ("pad", "<i4")])
 FORCE_BLOCK = 256

@@ -27,28 +27,50 @@ import numpy as np
 import torch
 
 from pylabolt_amd import _native as nt
+from pylabolt_amd.engine import ORIGIN
 
 N_XCD = 8
 POLICY_STATIC, POLICY_DYNAMIC = 0, 1
 
 
 def work_items(engines, tile_i):
-    """Per-lattice work items ``(lattice, code)`` and cell costs.
+    """Per-lattice work items ``(lattice, code)``.
 
-    code >= 0: position in the lattice's tile list (bulk tiles first);
-    code < 0: fix-up chunk ``-code - 1`` of ``64 * tile_i`` cells.  Within a
-    lattice the general tiles come first, then the fix-up chunks, then the
-    bulk tiles in row order (neighbouring tiles share halo rows in L2).
+    code >= 0: position in the lattice's tile list; code < 0: fix-up chunk
+    ``-code - 1`` of ``64 * tile_i`` cells.  Within a lattice the tiles go
+    in row-band order and each fix-up chunk right after the band holding
+    the last row it touches: its cells' cache lines were just fetched by
+    those tiles and are still in the XCD's L2.
     """
     per = []
     for l, e in enumerate(engines):
         t = e.tables
-        nchunk = -(-t.n_fix // (64 * tile_i))
-        codes = ([t.n_bulk + g for g in range(t.n_general)] +
-                 [-(c + 1) for c in range(nchunk)] +
-                 list(range(t.n_bulk)))
-        per.append(np.array([(l, c) for c in codes], dtype=np.int32)
-                   .reshape(-1, 2))
+        wg = 64 * tile_i
+        nchunk = -(-t.n_fix // wg)
+        ntiles = t.n_bulk + t.n_general
+        band = np.asarray(t.tile_list[:ntiles]) // t.ntj
+        by_band = np.argsort(band, kind="stable")
+        # last interior row (1-based) of each fix-up chunk; the list is
+        # sorted by row
+        rows = (np.asarray(t.fix_cells[:t.n_fix], dtype=np.int64) - ORIGIN) \
+            // e.pitch
+        last = [int(rows[min((c + 1) * wg, t.n_fix) - 1])
+                for c in range(nchunk)]
+        codes, ci, k = [], 0, 0
+        for b in range(t.nti):
+            while k < ntiles and band[by_band[k]] == b:
+                codes.append(int(by_band[k]))
+                k += 1
+            # rows up to (b + 1) * tile_i are swept; the chunk also reads
+            # one row beyond its last
+            while ci < nchunk and last[ci] + 1 <= (b + 1) * tile_i:
+                codes.append(-(ci + 1))
+                ci += 1
+        codes += [int(p) for p in by_band[k:]]
+        codes += [-(c + 1) for c in range(ci, nchunk)]
+        per.append(np.stack([np.full(len(codes), l, dtype=np.int32),
+                             np.asarray(codes, dtype=np.int32)], axis=1)
+                   if codes else np.zeros((0, 2), dtype=np.int32))
     return per
 
 

@@ -14,11 +14,12 @@ from pylabolt_amd.engine import CaseTables, elements_from_boundary
 
 
 def _fake_engine(sim, B, tile_i):
+    from pylabolt_amd.engine import geometry
     st = State(sim)
     nx, ny = (int(v) for v in sim.mesh_dict["grid"])
     t = CaseTables(nx, ny, st.fields.solid, elements_from_boundary(st.boundary),
                    st.boundary.x_periodic, st.boundary.y_periodic, B, tile_i)
-    return SimpleNamespace(nx=nx, ny=ny, tables=t)
+    return SimpleNamespace(nx=nx, ny=ny, tables=t, pitch=geometry(nx, ny)[0])
 
 
 @pytest.mark.parametrize("tile_i", [1, 2, 4])
@@ -50,6 +51,11 @@ def test_items_cover_tiles_and_fix_cells_once(tile_i):
         # the tile list covers every tile of the lattice once
         assert np.array_equal(np.sort(t.tile_list),
                               np.arange(t.nti * t.ntj))
+        # tiles in row-band order; a fix-up chunk follows the band of the
+        # row after its last row (or comes at the end)
+        own = per[l][:, 1]
+        bands = [int(t.tile_list[c]) // t.ntj for c in own if c >= 0]
+        assert bands == sorted(bands)
 
 
 @pytest.mark.parametrize("sizes", [
