@@ -158,6 +158,14 @@ typedef struct d2q9_lattice {
   const d2q9_bc_node* bc_nodes;
   const d2q9_bc_elem* bc_elems;
   const void* force_field;      /* [2][plane] when force_source == FIELD */
+  /* Zero-gradient rows swept inside bulk tiles (null: none).  For bulk
+   * tile p and row r, words [((p * tile_i + r) * 2 + s) * cpl, + cpl) mark
+   * (one bit per cell, as bulk_masks) the cells whose only boundary rule
+   * is zero-gradient from a wall across y: s = 0 takes directions 2,5,6
+   * from the cell at j + 1, s = 1 takes 4,7,8 from the cell at j - 1, both
+   * as pulled before any boundary rule.  Those cells are not masked in
+   * bulk_masks and not in the fix-up list. */
+  const uint64_t* zg_masks;
 } d2q9_lattice;
 
 /* Populations f = feq(rho0, u0) on fluid cells of both buffers, the encoded

@@ -575,10 +575,11 @@ int d2q9_abi_layout(int64_t* out, int n) {
       (int64_t)offsetof(d2q9_lattice, bc_slot),
       (int64_t)offsetof(d2q9_lattice, bc_nodes), (int64_t)offsetof(d2q9_lattice, bc_elems),
       (int64_t)offsetof(d2q9_lattice, force_field),
+      (int64_t)offsetof(d2q9_lattice, zg_masks),
       (int64_t)sizeof(d2q9_batch_entry), (int64_t)offsetof(d2q9_batch_entry, f),
       (int64_t)offsetof(d2q9_batch_entry, mom), (int64_t)sizeof(d2q9_batch_item)};
   const int m = (int)(sizeof(v) / sizeof(v[0]));
-  if (!out || n < m) return fail(D2Q9_E_ARG, "abi layout needs 28 slots");
+  if (!out || n < m) return fail(D2Q9_E_ARG, "abi layout needs 29 slots");
   for (int q = 0; q < m; ++q) out[q] = v[q];
   return m;
 }

@@ -187,6 +187,62 @@ __device__ __forceinline__ void store_block(
   }
 }
 
+// Zero-gradient wall rows inside a bulk tile: a marked cell takes the
+// unknown directions (2,5,6 from the cell at j + 1, or 4,7,8 from the cell
+// at j - 1) as that cell pulled them, before any boundary rule -- the same
+// values the rule reads (apply_bc, zero-gradient branch) when the source
+// cell has no solid neighbour and no periodic wrap, which the host checks.
+// The source is the next cell of the lane or, at a lane edge, of the
+// neighbouring lane (one shuffle).  The masks are uniform over the wave.
+template <int S>
+__device__ __forceinline__ void zero_gradient_rows(const d2q9_lattice& L,
+                                                   typename Codec<S>::T (&g)[Codec<S>::CPL][9],
+                                                   int p, int lane) {
+  using T = typename Codec<S>::T;
+  constexpr int CPL = Codec<S>::CPL;
+  const uint64_t* zrow = L.zg_masks + ((int64_t)p * blockDim.y + threadIdx.y) * 2 * CPL;
+  uint32_t up = 0, down = 0;
+  bool any_up = false, any_down = false;
+#pragma unroll
+  for (int w = 0; w < CPL; ++w) {
+    any_up |= zrow[w] != 0;
+    any_down |= zrow[CPL + w] != 0;
+  }
+  if (!any_up && !any_down) return;
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const int o = lane * CPL + q;
+    up |= (uint32_t)((zrow[o >> 6] >> (o & 63)) & 1ull) << q;
+    down |= (uint32_t)((zrow[CPL + (o >> 6)] >> (o & 63)) & 1ull) << q;
+  }
+  constexpr int DU[3] = {2, 5, 6};
+  constexpr int DD[3] = {4, 7, 8};
+  if (any_up) {
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const int d = DU[n];
+      const T nxt = __shfl_down(g[0][d], 1, 64);
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) {
+        const T src = (q + 1 < CPL) ? g[q + 1][d] : nxt;
+        if ((up >> q) & 1) g[q][d] = src;
+      }
+    }
+  }
+  if (any_down) {
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const int d = DD[n];
+      const T prv = __shfl_up(g[CPL - 1][d], 1, 64);
+#pragma unroll
+      for (int q = CPL - 1; q >= 0; --q) {
+        const T src = (q > 0) ? g[q - 1][d] : prv;
+        if ((down >> q) & 1) g[q][d] = src;
+      }
+    }
+  }
+}
+
 // Bulk tile at position p of the tile list (host-classified: no solid
 // cell).  Every cell is swept as plain interior fluid and no flag is read;
 // the cells marked in bulk_masks are not stored here: the special ones
@@ -224,6 +280,9 @@ __device__ __forceinline__ void bulk_tile(
   }
   T g[CPL][9];
   gather_block<S, PHASE>(L, K, fsrc, i, tj, lane, j0, g);
+  if constexpr (PHASE != PHASE_COLLIDE) {
+    if (L.zg_masks) zero_gradient_rows<S>(L, g, p, lane);
+  }
   typename C::raw out[9][CPL];
 #pragma unroll
   for (int q = 0; q < CPL; ++q)

@@ -79,6 +79,12 @@ def padded_view(flat, q, nx, ny, pitch, plane):
     return flat[base:base + (nx + 2) * pitch].reshape(nx + 2, pitch)[:, :ny + 2]
 
 
+def in_bulk_cells(tiles, tile_i, tile_j, nx, ny):
+    """Interior cells (nx, ny) lying in a bulk tile."""
+    return np.repeat(np.repeat(tiles, tile_i, axis=0), tile_j,
+                     axis=1)[:nx, :ny]
+
+
 class CaseTables:
     """Host-built per-cell flags, boundary rules and tile classes.
 
@@ -145,6 +151,20 @@ class CaseTables:
             self.bc_nodes[n, :9] = rules[key]
             self.bc_slot_pad[key] = n
         self.n_bc_nodes = len(keys)
+        # zero-gradient wall cells across y whose only rules are that wall's
+        # (side 0: normal (0, 1), directions 2,5,6 from j + 1; side 1:
+        # normal (0, -1), directions 4,7,8 from j - 1)
+        self.zg_side_pad = np.full((nxp, nyp), -1, dtype=np.int8)
+        for (i, j), r in rules.items():
+            dirs = [k for k in range(9) if r[k] & 15 != nt.RULE_STREAM]
+            if not dirs or any(r[k] & 15 != nt.RULE_ZG for k in dirs):
+                continue
+            normals = {tuple(int(v) for v in self.elems[r[k] >> 4][1]["normal"])
+                       for k in dirs}
+            if normals == {(0, 1)} and sorted(dirs) == [2, 5, 6]:
+                self.zg_side_pad[i, j] = 0
+            elif normals == {(0, -1)} and sorted(dirs) == [4, 7, 8]:
+                self.zg_side_pad[i, j] = 1
         self.set_tile(tile_j, tile_i)
 
     def set_tile(self, tile_j, tile_i):
@@ -173,6 +193,31 @@ class CaseTables:
             special[:, 0] = True
             special[:, -1] = True
         special &= fluid
+        # zero-gradient cells the bulk sweep handles itself (copy from the
+        # neighbour in y): plain boundary cells whose source is a plain pull
+        # of the same block, not itself such a cell
+        side = self.zg_side_pad[1:-1, 1:-1]
+        edge = np.zeros_like(fluid)
+        if self.x_periodic:
+            edge[0, :] = edge[-1, :] = True
+        if self.y_periodic:
+            edge[:, 0] = edge[:, -1] = True
+        plain_bc = (inner == (nt.CELL_FLUID | nt.CELL_BC)) & ~edge
+        cand = plain_bc & (side >= 0)
+        zg = np.zeros_like(fluid)
+        for s_, step in ((0, 1), (1, -1)):
+            ci, cj = np.nonzero(cand & (side == s_))
+            sj = cj + step
+            ok = (sj >= 0) & (sj < ny)
+            ci, cj, sj = ci[ok], cj[ok], sj[ok]
+            src_flags = inner[ci, sj]
+            ok = (((src_flags & nt.CELL_FLUID) != 0) &
+                  ((src_flags & nt.CELL_NEAR_SOLID) == 0) &
+                  ~edge[ci, sj] & ~cand[ci, sj] &
+                  (cj // self.tile_j == sj // self.tile_j))
+            zg[ci[ok], cj[ok]] = True
+        self.zg_inline = zg
+        special &= ~zg
         ntj = -(-ny // self.tile_j)
         nti = -(-nx // self.tile_i)
         # cells past the lattice count as fluid here: they do not stop a
@@ -202,6 +247,20 @@ class CaseTables:
         self.bulk_masks = np.ascontiguousarray(m[bulk_ids]).reshape(-1)
         if self.bulk_masks.size == 0:
             self.bulk_masks = np.zeros(1, dtype=np.uint64)
+        # zero-gradient bits per (bulk tile, row, side, word)
+        zm = []
+        for s_ in (0, 1):
+            zpad = np.zeros((nti * self.tile_i, ntj * self.tile_j), dtype=bool)
+            zpad[:nx, :ny] = zg & (side == s_)
+            zb = zpad.reshape(nti, self.tile_i, ntj, words, 64)
+            w_ = (zb.astype(np.uint64) * weights).sum(axis=-1, dtype=np.uint64)
+            zm.append(w_.transpose(0, 2, 1, 3).reshape(nti * ntj, self.tile_i,
+                                                       words))
+        zmask = np.stack(zm, axis=2)[bulk_ids]   # [bulk, row, side, word]
+        self.n_zg = int((zg & in_bulk_cells(tiles, self.tile_i, self.tile_j,
+                                            nx, ny)).sum())
+        self.zg_masks = (np.ascontiguousarray(zmask).reshape(-1)
+                         if self.n_zg else None)
         ii, jj = np.nonzero(special & in_bulk)
         pitch = int(math.ceil((ny + 2) / ALIGN) * ALIGN)
         self.fix_cells = (ORIGIN + (ii + 1) * pitch + (jj + 1)).astype(np.int32)
@@ -392,6 +451,13 @@ class D2Q9Engine:
         L.fix_cells = self.fix_cells.data_ptr()
         L.n_fix = self.tables.n_fix
         L.bulk_masks = self.bulk_masks.data_ptr()
+        self.zg_masks = None
+        if self.tables.zg_masks is not None:
+            with torch.cuda.stream(self.stream):
+                self.zg_masks = torch.from_numpy(
+                    self.tables.zg_masks.view(np.int64)).to(self.device)
+        L.zg_masks = (self.zg_masks.data_ptr() if self.zg_masks is not None
+                      else None)
 
     def set_tile_rows(self, tile_i):
         """Re-tile with ``tile_i`` rows per tile (workgroup 64 x tile_i);
