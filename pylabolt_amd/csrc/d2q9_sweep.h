@@ -69,13 +69,20 @@ __device__ __forceinline__ float wave_readlane<float>(float v, int lane) {
   return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v), lane));
 }
 
+// Raw 8-byte words of one lane's gather (one per direction) and, for
+// CPL > 1, the halo element; loaded by gather_issue, decoded by
+// gather_finish.
+template <int S>
+struct RawBlock {
+  uint64_t w[9];
+  typename Codec<S>::raw halo;
+};
+
 template <int S, int PHASE>
-__device__ __forceinline__ void gather_block(
-    const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K,
-    const typename Codec<S>::raw* __restrict__ fsrc, int i, int tj, int lane,
-    int j0, typename Codec<S>::T (&g)[Codec<S>::CPL][9]) {
+__device__ __forceinline__ void gather_issue(const d2q9_lattice& L,
+                                             const typename Codec<S>::raw* __restrict__ fsrc,
+                                             int i, int tj, int lane, int j0, RawBlock<S>& rb) {
   using C = Codec<S>;
-  using T = typename C::T;
   using R = typename C::raw;
   constexpr int CPL = C::CPL;
   constexpr int B = 64 * CPL;
@@ -85,16 +92,16 @@ __device__ __forceinline__ void gather_block(
       const int row = (PHASE == PHASE_COLLIDE) ? i : i - CX[k];
       const int cy = (PHASE == PHASE_COLLIDE) ? 0 : CY[k];
       const int jc = (j0 <= L.ny) ? j0 - cy : j0;
-      g[0][k] = C::dec(fsrc[pop_index<B>(L, row, jc, k)], k, K);
+      rb.w[k] = *reinterpret_cast<const uint64_t*>(fsrc + pop_index<B>(L, row, jc, k));
     }
   } else {
-    Vec8<R, CPL> v[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const int row = (PHASE == PHASE_COLLIDE) ? i : i - CX[k];
-      v[k] = load8<R, CPL>(fsrc + (((int64_t)row * L.nbj + tj + 1) * 9 + k) * B + lane * CPL);
+      rb.w[k] = *reinterpret_cast<const uint64_t*>(
+          fsrc + (((int64_t)row * L.nbj + tj + 1) * 9 + k) * B + lane * CPL);
     }
-    R halo = R();
+    rb.halo = R();
     if (PHASE != PHASE_COLLIDE) {
       // lane h < 6 fetches the end element of direction HK[h]
       const int h = lane < 6 ? lane : 0;
@@ -103,20 +110,41 @@ __device__ __forceinline__ void gather_block(
       const int jf = 1 + tj * B;             // first cell of the block
       int jh = h < 3 ? jf - 1 : jf + B;      // element before / after
       if (jh > L.ny + 1) jh = jf;            // past the padded row: unused
-      halo = fsrc[pop_index<B>(L, i - cxh, jh, kh)];
+      rb.halo = fsrc[pop_index<B>(L, i - cxh, jh, kh)];
     }
+  }
+}
+
+template <int S, int PHASE>
+__device__ __forceinline__ void gather_finish(const Consts<typename Codec<S>::T>& K,
+                                              const RawBlock<S>& rb, int lane,
+                                              typename Codec<S>::T (&g)[Codec<S>::CPL][9]) {
+  using C = Codec<S>;
+  using T = typename C::T;
+  using R = typename C::raw;
+  constexpr int CPL = C::CPL;
+  if constexpr (CPL == 1) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      R r;
+      __builtin_memcpy(&r, &rb.w[k], sizeof(R));
+      g[0][k] = C::dec(r, k, K);
+    }
+  } else {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const int cy = (PHASE == PHASE_COLLIDE) ? 0 : CY[k];
+      Vec8<R, CPL> v;
+      __builtin_memcpy(&v, &rb.w[k], 8);
       T x[CPL];
 #pragma unroll
-      for (int q = 0; q < CPL; ++q) x[q] = C::dec(v[k].v[q], k, K);
+      for (int q = 0; q < CPL; ++q) x[q] = C::dec(v.v[q], k, K);
       if (cy == 0) {
 #pragma unroll
         for (int q = 0; q < CPL; ++q) g[q][k] = x[q];
       } else {
         const int h = k == 2 ? 0 : k == 5 ? 1 : k == 6 ? 2 : k == 4 ? 3 : k == 7 ? 4 : 5;
-        const T hv = wave_readlane<T>(C::dec(halo, k, K), h);
+        const T hv = wave_readlane<T>(C::dec(rb.halo, k, K), h);
         if (cy == 1) {  // cell j takes j - 1
           T prev = __shfl_up(x[CPL - 1], 1, 64);
           g[0][k] = lane == 0 ? hv : prev;
@@ -131,6 +159,16 @@ __device__ __forceinline__ void gather_block(
       }
     }
   }
+}
+
+template <int S, int PHASE>
+__device__ __forceinline__ void gather_block(
+    const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K,
+    const typename Codec<S>::raw* __restrict__ fsrc, int i, int tj, int lane,
+    int j0, typename Codec<S>::T (&g)[Codec<S>::CPL][9]) {
+  RawBlock<S> rb;
+  gather_issue<S, PHASE>(L, fsrc, i, tj, lane, j0, rb);
+  gather_finish<S, PHASE>(K, rb, lane, g);
 }
 
 // Moments + collision (or nothing for PHASE_STREAM) of one cell, encoded
@@ -249,25 +287,28 @@ __device__ __forceinline__ void zero_gradient_rows(const d2q9_lattice& L,
 // (solid neighbour, boundary rule, stored moment, periodic edge) are
 // written by the fix-up pass, the ones past the lattice's last column are
 // never written.
-template <int S, int MODEL, int FORCING, int PHASE>
-__device__ __forceinline__ void bulk_tile(
-    const d2q9_lattice& L, const typename Codec<S>::raw* __restrict__ fsrc,
-    typename Codec<S>::raw* __restrict__ fdst,
-    typename Codec<S>::T* __restrict__ mdst, int write_mom, int ntj, int p) {
-  using C = Codec<S>;
-  using T = typename C::T;
-  constexpr int CPL = C::CPL;
+// Position of bulk tile p for this wave: row i, block tj, first cell j0 of
+// the lane, and the lane's masked cells.
+struct BulkPos {
+  int i, tj, j0;
+  uint32_t skip;
+};
+
+template <int S>
+__device__ __forceinline__ BulkPos bulk_pos(const d2q9_lattice& L, int ntj, int p) {
+  constexpr int CPL = Codec<S>::CPL;
   constexpr int B = 64 * CPL;
+  BulkPos bp;
   const int t = L.tiles[p];
-  const int tj = t % ntj, ti = t / ntj;
+  bp.tj = t % ntj;
+  const int ti = t / ntj;
   const int lane = (int)threadIdx.x;
-  const int i = 1 + ti * (int)blockDim.y + (int)threadIdx.y;
-  const int j0 = 1 + tj * B + lane * CPL;
-  if (i > L.nx) return;  // whole wave (last tile row of a partial tile)
-  const Consts<T> K = load_consts<T>(L);
-  // special bits of this lane's cells
+  bp.i = 1 + ti * (int)blockDim.y + (int)threadIdx.y;
+  bp.j0 = 1 + bp.tj * B + lane * CPL;
+  bp.skip = 0;
+  if (bp.i > L.nx) return bp;  // whole wave (last tile row of a partial tile)
+  // masked bits of this lane's cells
   const uint64_t* mrow = L.bulk_masks + ((int64_t)p * blockDim.y + threadIdx.y) * CPL;
-  uint32_t skip = 0;
   bool any = false;
 #pragma unroll
   for (int w = 0; w < CPL; ++w) any |= mrow[w] != 0;
@@ -275,29 +316,64 @@ __device__ __forceinline__ void bulk_tile(
 #pragma unroll
     for (int q = 0; q < CPL; ++q) {
       const int o = lane * CPL + q;
-      skip |= (uint32_t)((mrow[o >> 6] >> (o & 63)) & 1ull) << q;
+      bp.skip |= (uint32_t)((mrow[o >> 6] >> (o & 63)) & 1ull) << q;
     }
   }
+  return bp;
+}
+
+// Everything after the loads of one bulk tile: decode and shuffle, the
+// zero-gradient rows, moments + collision, the stores.
+template <int S, int MODEL, int FORCING, int PHASE>
+__device__ __forceinline__ void bulk_finish(
+    const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K, const RawBlock<S>& rb,
+    const BulkPos& bp, typename Codec<S>::raw* __restrict__ fdst,
+    typename Codec<S>::T* __restrict__ mdst, int write_mom, int p) {
+  using C = Codec<S>;
+  using T = typename C::T;
+  constexpr int CPL = C::CPL;
+  constexpr int B = 64 * CPL;
+  const int lane = (int)threadIdx.x;
   T g[CPL][9];
-  gather_block<S, PHASE>(L, K, fsrc, i, tj, lane, j0, g);
+  gather_finish<S, PHASE>(K, rb, lane, g);
   if constexpr (PHASE != PHASE_COLLIDE) {
     if (L.zg_masks) zero_gradient_rows<S>(L, g, p, lane);
   }
   typename C::raw out[9][CPL];
 #pragma unroll
   for (int q = 0; q < CPL; ++q)
-    finish_cell<S, MODEL, FORCING, PHASE>(L, K, g[q], out, q, cell_index(L, i, j0 + q),
-                                          write_mom != 0 && !((skip >> q) & 1), mdst);
-  if (skip == 0) {
-    store_block<S>(L, fdst, i, tj, lane, out);
+    finish_cell<S, MODEL, FORCING, PHASE>(L, K, g[q], out, q, cell_index(L, bp.i, bp.j0 + q),
+                                          write_mom != 0 && !((bp.skip >> q) & 1), mdst);
+  if (bp.skip == 0) {
+    store_block<S>(L, fdst, bp.i, bp.tj, lane, out);
   } else {
 #pragma unroll
     for (int q = 0; q < CPL; ++q) {
-      if ((skip >> q) & 1) continue;
+      if ((bp.skip >> q) & 1) continue;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) fdst[pop_index<B>(L, i, j0 + q, k)] = out[k][q];
+      for (int k = 0; k < 9; ++k) fdst[pop_index<B>(L, bp.i, bp.j0 + q, k)] = out[k][q];
     }
   }
+}
+
+// Bulk tile at position p of the tile list (host-classified: no solid
+// cell).  Every cell is swept as plain interior fluid and no flag is read;
+// the cells marked in bulk_masks are not stored here: the special ones
+// (solid neighbour, boundary rule, stored moment, periodic edge) are
+// written by the fix-up pass, the ones past the lattice's last column are
+// never written.
+template <int S, int MODEL, int FORCING, int PHASE>
+__device__ __forceinline__ void bulk_tile(
+    const d2q9_lattice& L, const typename Codec<S>::raw* __restrict__ fsrc,
+    typename Codec<S>::raw* __restrict__ fdst,
+    typename Codec<S>::T* __restrict__ mdst, int write_mom, int ntj, int p) {
+  using T = typename Codec<S>::T;
+  const BulkPos bp = bulk_pos<S>(L, ntj, p);
+  if (bp.i > L.nx) return;
+  const Consts<T> K = load_consts<T>(L);
+  RawBlock<S> rb;
+  gather_issue<S, PHASE>(L, fsrc, bp.i, bp.tj, (int)threadIdx.x, bp.j0, rb);
+  bulk_finish<S, MODEL, FORCING, PHASE>(L, K, rb, bp, fdst, mdst, write_mom, p);
 }
 
 // Fix-up cells [first, first + workgroup size) of the lattice's list, one
