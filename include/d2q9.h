@@ -53,9 +53,9 @@ extern "C" {
 /* storage of the populations in HBM */
 enum {
   D2Q9_STORE_F64 = 0,     /* double, compute in double (the parity path) */
-  D2Q9_STORE_F32 = 1,     /* float of (f_i - w_i) * dev_scale_i, compute float */
+  D2Q9_STORE_F32 = 1,     /* float of (f_i - w_i) (dev_scale 1), compute float */
   D2Q9_STORE_F16DEV = 2,  /* half of (f_i - w_i) * dev_scale_i, compute float */
-  D2Q9_STORE_BF16DEV = 3  /* bfloat16 of the same, compute float */
+  D2Q9_STORE_BF16DEV = 3  /* bfloat16 of (f_i - w_i) (dev_scale 1), compute float */
 };
 enum { D2Q9_BGK = 0, D2Q9_MRT = 1 };
 enum { D2Q9_FORCING_NONE = 0, D2Q9_GUO_LINEAR = 1, D2Q9_GUO_SECOND_ORDER = 2 };

@@ -36,6 +36,8 @@
 #include <hip/hip_bf16.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/d2q9.h"
 
 namespace d2q9 {
@@ -129,9 +131,14 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
 // Random bits for rounding value `b` (float bits) of direction k of cell
 // `salt`: a function of the data only, so a run is reproducible bit for
 // bit, while the rounding of a slowly changing field is decorrelated from
-// step to step.
-__device__ __forceinline__ uint32_t round_bits(uint32_t b, int k, uint32_t salt) {
-  return mix32(b ^ ((salt * 9u + (uint32_t)k) * 0x9e3779b9u));
+// step to step.  The cell's key is hashed once per cell (cell_key); per
+// value one 24-bit multiply (full rate) spreads the low 24 bits of
+// value ^ key into bits 16..31, of which the caller keeps the low 13 (half)
+// or 16 (bfloat16).
+__device__ __forceinline__ uint32_t cell_key(uint32_t salt) { return mix32(salt); }
+__device__ __forceinline__ uint32_t round_bits(uint32_t b, int k, uint32_t key) {
+  const uint32_t x = b ^ (key + (uint32_t)k * 0x632be5abu);
+  return __umul24(x, 0x9e3779u) >> 16;
 }
 
 template <int S> struct Codec;
@@ -147,26 +154,28 @@ template <> struct Codec<D2Q9_STORE_F64> {
   static __device__ __forceinline__ double f64(raw r, int, const d2q9_lattice&) { return r; }
 };
 
+// FP32 and BF16 store d unscaled (dev_scale must be 1); FP16 scales it.
 template <> struct Codec<D2Q9_STORE_F32> {
   using raw = float;
   using T = float;
   static constexpr bool dev = true;
   static constexpr int CPL = 2;
-  static __device__ __forceinline__ T dec(raw r, int k, const Consts<T>& K) { return r * K.iscale[k]; }
-  static __device__ __forceinline__ raw enc(T v, int k, const Consts<T>& K) { return v * K.scale[k]; }
-  static __device__ __forceinline__ raw encs(T v, int k, uint32_t, const Consts<T>& K) {
-    return enc(v, k, K);
-  }
+  static __device__ __forceinline__ T dec(raw r, int, const Consts<T>&) { return r; }
+  static __device__ __forceinline__ raw enc(T v, int, const Consts<T>&) { return v; }
+  static __device__ __forceinline__ raw encs(T v, int, uint32_t, const Consts<T>&) { return v; }
   static __device__ __forceinline__ double f64(raw r, int k, const d2q9_lattice& L) {
-    return L.w[k] + (double)r / L.dev_scale[k];
+    return L.w[k] + (double)r;
   }
 };
+
+// cells per lane of the 16-bit forms (one lane moves 4 bytes per direction)
+constexpr int CPL16 = 2;
 
 template <> struct Codec<D2Q9_STORE_F16DEV> {
   using raw = __half;
   using T = float;
   static constexpr bool dev = true;
-  static constexpr int CPL = 4;
+  static constexpr int CPL = CPL16;
   static __device__ __forceinline__ T dec(raw r, int k, const Consts<T>& K) {
     return __half2float(r) * K.iscale[k];
   }
@@ -176,9 +185,9 @@ template <> struct Codec<D2Q9_STORE_F16DEV> {
   // Stochastic rounding to half: random bits added below the 10 kept
   // mantissa bits, then truncation.  The truncated float is exact in half
   // over half's normal range; below it the conversion rounds to nearest.
-  static __device__ __forceinline__ raw encs(T v, int k, uint32_t salt, const Consts<T>& K) {
+  static __device__ __forceinline__ raw encs(T v, int k, uint32_t key, const Consts<T>& K) {
     const uint32_t b = __float_as_uint(v * K.scale[k]);
-    const uint32_t t = (b + (round_bits(b, k, salt) & 0x1fffu)) & ~0x1fffu;
+    const uint32_t t = (b + (round_bits(b, k, key) & 0x1fffu)) & ~0x1fffu;
     return __float2half_rn(__uint_as_float(t));
   }
   static __device__ __forceinline__ double f64(raw r, int k, const d2q9_lattice& L) {
@@ -190,21 +199,21 @@ template <> struct Codec<D2Q9_STORE_BF16DEV> {
   using raw = __hip_bfloat16;
   using T = float;
   static constexpr bool dev = true;
-  static constexpr int CPL = 4;
-  static __device__ __forceinline__ T dec(raw r, int k, const Consts<T>& K) {
-    return __bfloat162float(r) * K.iscale[k];
+  static constexpr int CPL = CPL16;
+  static __device__ __forceinline__ T dec(raw r, int, const Consts<T>&) {
+    return __bfloat162float(r);
   }
-  static __device__ __forceinline__ raw enc(T v, int k, const Consts<T>& K) {
-    return __float2bfloat16(v * K.scale[k]);
+  static __device__ __forceinline__ raw enc(T v, int, const Consts<T>&) {
+    return __float2bfloat16(v);
   }
   // Stochastic rounding to bfloat16 (the upper 16 bits of a float).
-  static __device__ __forceinline__ raw encs(T v, int k, uint32_t salt, const Consts<T>& K) {
-    const uint32_t b = __float_as_uint(v * K.scale[k]);
-    const uint32_t t = b + (round_bits(b, k, salt) & 0xffffu);
+  static __device__ __forceinline__ raw encs(T v, int k, uint32_t key, const Consts<T>&) {
+    const uint32_t b = __float_as_uint(v);
+    const uint32_t t = b + (round_bits(b, k, key) & 0xffffu);
     return __ushort_as_bfloat16((unsigned short)(t >> 16));
   }
   static __device__ __forceinline__ double f64(raw r, int k, const d2q9_lattice& L) {
-    return L.w[k] + (double)__bfloat162float(r) / L.dev_scale[k];
+    return L.w[k] + (double)__bfloat162float(r);
   }
 };
 
@@ -295,7 +304,13 @@ __device__ __forceinline__ void moments(const T (&g)[9], const d2q9_lattice& L,
     sx += (T)CX[k] * g[k];
     sy += (T)CY[k] * g[k];
   }
-  T inv = (T)1 / (rho + K.fmin);
+  // FP64 divides as the reference does; the float forms use the
+  // hardware reciprocal (1 ulp)
+  T inv;
+  if constexpr (std::is_same<T, double>::value)
+    inv = (T)1 / (rho + K.fmin);
+  else
+    inv = __builtin_amdgcn_rcpf(rho + K.fmin);
   if (FORCING != D2Q9_FORCING_NONE || field) {
     ux = sx * inv + ((T)0.5 * fx) * inv;
     uy = sy * inv + ((T)0.5 * fy) * inv;
@@ -307,10 +322,63 @@ __device__ __forceinline__ void moments(const T (&g)[9], const d2q9_lattice& L,
   }
 }
 
+// Float BGK (deviation form) over the four opposite-direction pairs: with
+// cu_kb = -cu_k, the equilibrium deviation splits into a part shared by
+// the pair, w (drho + rho (0.5 ics4 cu^2 - 0.5 ics2 u^2)), and one of
+// opposite sign, w rho ics2 cu; the Guo source likewise into
+// w (ics4 cu (c.F) - ics2 (u.F)) and w ics2 (c.F) (second order) or 0 and
+// w ics2 (c.F) (linear).  g <- g + om (geq - g) (+ (1 - om/2) Fbar).
+template <int FORCING>
+__device__ __forceinline__ void collide_bgk_pairs(float (&g)[9], const Consts<float>& K,
+                                                  float drho, float rho, float ux, float uy,
+                                                  float fx, float fy) {
+  const float om = K.om;
+  const float ics2 = K.ics2;
+  const float hics4 = 0.5f * K.ics4;
+  const float hu2 = (0.5f * ics2) * (ux * ux + uy * uy);
+  const float rics2 = rho * ics2;
+  const float cf = 1.0f - 0.5f * om;
+  const float uf = ics2 * (ux * fx + uy * fy);
+  // rest direction (cu = 0, c = 0)
+  {
+    float eq = K.w[0] * (drho - rho * hu2);
+    float v = g[0] + om * (eq - g[0]);
+    if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) v += cf * (K.w[0] * (-uf));
+    g[0] = v;
+  }
+  constexpr int PK[4] = {1, 2, 5, 6};
+  constexpr int PB[4] = {3, 4, 7, 8};
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const int k = PK[n], kb = PB[n];
+    const float cu = (float)CX[k] * ux + (float)CY[k] * uy;
+    const float w = K.w[k];
+    const float sym = drho + rho * (hics4 * cu * cu - hu2);
+    const float asym = rics2 * cu;
+    const float eqk = w * (sym + asym), eqb = w * (sym - asym);
+    float vk = g[k] + om * (eqk - g[k]);
+    float vb = g[kb] + om * (eqb - g[kb]);
+    if constexpr (FORCING != D2Q9_FORCING_NONE) {
+      const float cF = (float)CX[k] * fx + (float)CY[k] * fy;
+      const float fa = w * (ics2 * cF);
+      float fs = 0.0f;
+      if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) fs = w * ((2.0f * hics4) * cu * cF - uf);
+      vk += cf * (fs + fa);
+      vb += cf * (fs - fa);
+    }
+    g[k] = vk;
+    g[kb] = vb;
+  }
+}
+
 // BGK collision in place: g <- (1 - om) g + om geq (+ (1 - om/2) Fbar).
 template <typename T, int FORCING, bool DEV>
 __device__ __forceinline__ void collide_bgk(T (&g)[9], const Consts<T>& K,
                                             T drho, T rho, T ux, T uy, T fx, T fy) {
+  if constexpr (std::is_same<T, float>::value && DEV) {
+    collide_bgk_pairs<FORCING>(g, K, drho, rho, ux, uy, fx, fy);
+    return;
+  }
   const T om = K.om;
   const T ics2 = K.ics2, ics4 = K.ics4;
   const T u2 = ux * ux + uy * uy;

@@ -245,9 +245,13 @@ int validate(const d2q9_lattice* L) {
   if (L->tile_i < 1 || L->tile_i > 4) return fail(D2Q9_E_ARG, "tile_i must be 1..4");
   if (L->storage < 0 || L->storage > 3) return fail(D2Q9_E_ARG, "bad storage");
   {
-    const int cpl = L->storage == D2Q9_STORE_F64 ? 1 : (L->storage == D2Q9_STORE_F32 ? 2 : 4);
+    const int cpl = L->storage == D2Q9_STORE_F64 ? 1 : (L->storage == D2Q9_STORE_F32 ? 2 : CPL16);
     const int B = 64 * cpl;
     if (L->cpl != cpl) return fail(D2Q9_E_ARG, "cpl does not match the storage");
+    if (L->storage == D2Q9_STORE_F32 || L->storage == D2Q9_STORE_BF16DEV)
+      for (int k = 0; k < 9; ++k)
+        if (L->dev_scale[k] != 1.0 || L->cf.scale[k] != 1.0f)
+          return fail(D2Q9_E_ARG, "FP32 and BF16 storage keep dev_scale 1");
     if (L->tile_j != B) return fail(D2Q9_E_ARG, "tile_j must be 64 * cpl");
     if (L->nbj < (L->ny + B) / B + 1) return fail(D2Q9_E_ARG, "nbj too small");
     if (L->pop_len < (int64_t)(L->nx + 2) * L->nbj * 9 * B)

@@ -9,26 +9,23 @@ namespace d2q9 {
 
 enum { PHASE_COLLIDE = 0, PHASE_FUSED = 1, PHASE_STREAM = 2 };
 
-// 8-byte vector of CPL raw elements (one lane's share of one direction).
+// One lane's share of one direction: CPL raw elements moved as a single
+// 4- or 8-byte word.
+template <int BYTES> struct LaneWord;
+template <> struct LaneWord<4> { using type = uint32_t; };
+template <> struct LaneWord<8> { using type = uint64_t; };
+
 template <typename R, int CPL>
-struct alignas(8) Vec8 {
+struct alignas(sizeof(R) * CPL) Vec8 {
   R v[CPL];
 };
 
 template <typename R, int CPL>
-__device__ __forceinline__ Vec8<R, CPL> load8(const R* p) {
-  static_assert(sizeof(R) * CPL == 8, "one lane moves 8 bytes");
-  uint64_t w = *reinterpret_cast<const uint64_t*>(p);
-  Vec8<R, CPL> r;
-  __builtin_memcpy(&r, &w, 8);
-  return r;
-}
-
-template <typename R, int CPL>
 __device__ __forceinline__ void store8_nt(R* p, const Vec8<R, CPL>& v) {
-  uint64_t w;
-  __builtin_memcpy(&w, &v, 8);
-  __builtin_nontemporal_store(w, reinterpret_cast<uint64_t*>(p));
+  using W = typename LaneWord<sizeof(R) * CPL>::type;
+  W w;
+  __builtin_memcpy(&w, &v, sizeof(W));
+  __builtin_nontemporal_store(w, reinterpret_cast<W*>(p));
 }
 
 // Pre-collision values of one flagged cell: bounce-back off solid
@@ -69,12 +66,12 @@ __device__ __forceinline__ float wave_readlane<float>(float v, int lane) {
   return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v), lane));
 }
 
-// Raw 8-byte words of one lane's gather (one per direction) and, for
-// CPL > 1, the halo element; loaded by gather_issue, decoded by
-// gather_finish.
+// Raw words of one lane's gather (one per direction) and, for CPL > 1,
+// the halo element; loaded by gather_issue, decoded by gather_finish.
 template <int S>
 struct RawBlock {
-  uint64_t w[9];
+  using W = typename LaneWord<sizeof(typename Codec<S>::raw) * Codec<S>::CPL>::type;
+  W w[9];
   typename Codec<S>::raw halo;
 };
 
@@ -92,13 +89,14 @@ __device__ __forceinline__ void gather_issue(const d2q9_lattice& L,
       const int row = (PHASE == PHASE_COLLIDE) ? i : i - CX[k];
       const int cy = (PHASE == PHASE_COLLIDE) ? 0 : CY[k];
       const int jc = (j0 <= L.ny) ? j0 - cy : j0;
-      rb.w[k] = *reinterpret_cast<const uint64_t*>(fsrc + pop_index<B>(L, row, jc, k));
+      rb.w[k] = *reinterpret_cast<const typename RawBlock<S>::W*>(
+          fsrc + pop_index<B>(L, row, jc, k));
     }
   } else {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const int row = (PHASE == PHASE_COLLIDE) ? i : i - CX[k];
-      rb.w[k] = *reinterpret_cast<const uint64_t*>(
+      rb.w[k] = *reinterpret_cast<const typename RawBlock<S>::W*>(
           fsrc + (((int64_t)row * L.nbj + tj + 1) * 9 + k) * B + lane * CPL);
     }
     rb.halo = R();
@@ -135,7 +133,7 @@ __device__ __forceinline__ void gather_finish(const Consts<typename Codec<S>::T>
     for (int k = 0; k < 9; ++k) {
       const int cy = (PHASE == PHASE_COLLIDE) ? 0 : CY[k];
       Vec8<R, CPL> v;
-      __builtin_memcpy(&v, &rb.w[k], 8);
+      __builtin_memcpy(&v, &rb.w[k], sizeof(v));
       T x[CPL];
 #pragma unroll
       for (int q = 0; q < CPL; ++q) x[q] = C::dec(v.v[q], k, K);
@@ -194,8 +192,9 @@ __device__ __forceinline__ void finish_cell(
     moments<T, FORCING, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
     collide<T, MODEL, FORCING, C::dev>(g, K, drho, rho, ux, uy, fx, fy);
     // 16-bit storage rounds stochastically (deterministic in the data)
+    const uint32_t key = C::dev && sizeof(typename C::raw) == 2 ? cell_key((uint32_t)c) : 0u;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) out[k][q] = C::encs(g[k], k, (uint32_t)c, K);
+    for (int k = 0; k < 9; ++k) out[k][q] = C::encs(g[k], k, key, K);
     if (store_mom) {
       mdst[c] = rho;
       mdst[L.plane + c] = ux;

@@ -55,8 +55,8 @@ def geometry(nx, ny):
     return pitch, plane
 
 
-CPL = {nt.STORE_F64: 1, nt.STORE_F32: 2, nt.STORE_F16DEV: 4,
-       nt.STORE_BF16DEV: 4}
+CPL = {nt.STORE_F64: 1, nt.STORE_F32: 2, nt.STORE_F16DEV: 2,
+       nt.STORE_BF16DEV: 2}
 
 
 def pop_geometry(nx, ny, cpl):
