@@ -275,11 +275,48 @@ __device__ __forceinline__ T guo_k(int k, T ux, T uy, T fx, T fy, T wk,
 // Moments of the pre-collision values g (f, or d in deviation form):
 // rho = sum g (from 0) [+ 1 in deviation form], F = rho * gravity or the
 // per-cell field, u = (sum c g) * inv + (0.5 F) * inv, inv = 1/(rho + float_min).
+// Float moments (deviation form): opposite-direction sums, written-out
+// fused multiply-adds and the hardware reciprocal.
+template <int FORCING>
+__device__ __forceinline__ void moments_f32(const float (&g)[9], const d2q9_lattice& L,
+                                            const Consts<float>& K, float ffx, float ffy,
+                                            float& drho, float& rho, float& ux, float& uy,
+                                            float& fx, float& fy) {
+  // pairs of opposite directions first: fewer dependent adds
+  const float s13 = g[1] + g[3], s24 = g[2] + g[4], s57 = g[5] + g[7], s68 = g[6] + g[8];
+  const float d13 = g[1] - g[3], d24 = g[2] - g[4], d57 = g[5] - g[7], d68 = g[6] - g[8];
+  drho = (g[0] + s13) + (s24 + (s57 + s68));
+  rho = 1.0f + drho;
+  const bool field = L.force_source == D2Q9_FSRC_FIELD;
+  if (field) {
+    fx = ffx;
+    fy = ffy;
+  } else {
+    fx = rho * K.gx;
+    fy = rho * K.gy;
+  }
+  // c_x: +1 for 1,5,8; -1 for 3,6,7.  c_y: +1 for 2,5,6; -1 for 4,7,8.
+  const float sx = d13 + (d57 - d68);
+  const float sy = d24 + (d57 + d68);
+  const float inv = __builtin_amdgcn_rcpf(rho + K.fmin);
+  if (FORCING != D2Q9_FORCING_NONE || field) {
+    ux = __builtin_fmaf(0.5f, fx, sx) * inv;
+    uy = __builtin_fmaf(0.5f, fy, sy) * inv;
+  } else {
+    ux = sx * inv;
+    uy = sy * inv;
+  }
+}
+
 template <typename T, int FORCING, bool DEV>
 __device__ __forceinline__ void moments(const T (&g)[9], const d2q9_lattice& L,
                                         const Consts<T>& K, T ffx, T ffy,
                                         T& drho, T& rho, T& ux, T& uy, T& fx,
                                         T& fy) {
+  if constexpr (std::is_same<T, float>::value && DEV) {
+    moments_f32<FORCING>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
+    return;
+  }
   T r = (T)0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) r += g[k];
@@ -332,18 +369,20 @@ template <int FORCING>
 __device__ __forceinline__ void collide_bgk_pairs(float (&g)[9], const Consts<float>& K,
                                                   float drho, float rho, float ux, float uy,
                                                   float fx, float fy) {
+  // fused multiply-adds are written out (never left to the compiler), so
+  // every kernel build computes the same values
   const float om = K.om;
   const float ics2 = K.ics2;
   const float hics4 = 0.5f * K.ics4;
-  const float hu2 = (0.5f * ics2) * (ux * ux + uy * uy);
+  const float hu2 = (0.5f * ics2) * __builtin_fmaf(ux, ux, uy * uy);
   const float rics2 = rho * ics2;
-  const float cf = 1.0f - 0.5f * om;
-  const float uf = ics2 * (ux * fx + uy * fy);
+  const float cf = __builtin_fmaf(-0.5f, om, 1.0f);
+  const float uf = ics2 * __builtin_fmaf(ux, fx, uy * fy);
   // rest direction (cu = 0, c = 0)
   {
-    float eq = K.w[0] * (drho - rho * hu2);
-    float v = g[0] + om * (eq - g[0]);
-    if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) v += cf * (K.w[0] * (-uf));
+    const float eq = K.w[0] * __builtin_fmaf(-rho, hu2, drho);
+    float v = __builtin_fmaf(om, eq - g[0], g[0]);
+    if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) v = __builtin_fmaf(cf, -(K.w[0] * uf), v);
     g[0] = v;
   }
   constexpr int PK[4] = {1, 2, 5, 6};
@@ -353,18 +392,19 @@ __device__ __forceinline__ void collide_bgk_pairs(float (&g)[9], const Consts<fl
     const int k = PK[n], kb = PB[n];
     const float cu = (float)CX[k] * ux + (float)CY[k] * uy;
     const float w = K.w[k];
-    const float sym = drho + rho * (hics4 * cu * cu - hu2);
+    const float sym = __builtin_fmaf(rho, __builtin_fmaf(hics4 * cu, cu, -hu2), drho);
     const float asym = rics2 * cu;
     const float eqk = w * (sym + asym), eqb = w * (sym - asym);
-    float vk = g[k] + om * (eqk - g[k]);
-    float vb = g[kb] + om * (eqb - g[kb]);
+    float vk = __builtin_fmaf(om, eqk - g[k], g[k]);
+    float vb = __builtin_fmaf(om, eqb - g[kb], g[kb]);
     if constexpr (FORCING != D2Q9_FORCING_NONE) {
       const float cF = (float)CX[k] * fx + (float)CY[k] * fy;
-      const float fa = w * (ics2 * cF);
+      const float fa = (w * ics2) * cF;
       float fs = 0.0f;
-      if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) fs = w * ((2.0f * hics4) * cu * cF - uf);
-      vk += cf * (fs + fa);
-      vb += cf * (fs - fa);
+      if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER)
+        fs = w * __builtin_fmaf(K.ics4 * cu, cF, -uf);
+      vk = __builtin_fmaf(cf, fs + fa, vk);
+      vb = __builtin_fmaf(cf, fs - fa, vb);
     }
     g[k] = vk;
     g[kb] = vb;
