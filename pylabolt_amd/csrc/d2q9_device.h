@@ -435,6 +435,99 @@ __device__ __forceinline__ void collide_bgk(T (&g)[9], const Consts<T>& K,
   }
 }
 
+// The nine moments (rows of M) of a population-like vector given as its
+// rest value x0 and the sums S and differences D of the opposite pairs
+// (1,3), (2,4), (5,7), (6,8).
+__device__ __forceinline__ void moments_of_pairs(float x0, const float (&S)[4],
+                                                 const float (&D)[4], float (&m)[9]) {
+  const float s12 = S[0] + S[1], s34 = S[2] + S[3];
+  m[0] = x0 + s12 + s34;
+  m[1] = __builtin_fmaf(2.0f, s34, __builtin_fmaf(-4.0f, x0, -s12));
+  m[2] = __builtin_fmaf(-2.0f, s12, __builtin_fmaf(4.0f, x0, s34));
+  const float a = D[2] - D[3], b = D[2] + D[3];
+  m[3] = D[0] + a;
+  m[4] = __builtin_fmaf(-2.0f, D[0], a);
+  m[5] = D[1] + b;
+  m[6] = __builtin_fmaf(-2.0f, D[1], b);
+  m[7] = S[0] - S[1];
+  m[8] = S[2] - S[3];
+}
+
+// Float MRT (deviation form) with the same pair structure: the columns of
+// M for k and its opposite kb share the rows 0, 1, 2, 7, 8 and negate the
+// rows 3..6, so M (g - geq), M Fbar and the inverse map M^T diag(1/norm)
+// all go through pair sums and differences.  Same operator as the FP64
+// path, f* = f - M^-1 [S (M f - M feq) - (I - S/2) M Fbar], different
+// rounding.
+template <int FORCING>
+__device__ __forceinline__ void collide_mrt_pairs(float (&g)[9], const Consts<float>& K,
+                                                  float drho, float rho, float ux, float uy,
+                                                  float fx, float fy) {
+  const float ics2 = K.ics2;
+  const float hics4 = 0.5f * K.ics4;
+  const float hu2 = (0.5f * ics2) * __builtin_fmaf(ux, ux, uy * uy);
+  const float rics2 = rho * ics2;
+  const float uf = ics2 * __builtin_fmaf(ux, fx, uy * fy);
+  constexpr int PK[4] = {1, 2, 5, 6};
+  constexpr int PB[4] = {3, 4, 7, 8};
+  // non-equilibrium part e = g - geq and the Guo source, as pair sums and
+  // differences
+  const float e0 = g[0] - K.w[0] * __builtin_fmaf(-rho, hu2, drho);
+  float eS[4], eD[4], fS[4], fD[4];
+  float f0 = 0.0f;
+  if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) f0 = -(K.w[0] * uf);
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const int k = PK[n], kb = PB[n];
+    const float cu = (float)CX[k] * ux + (float)CY[k] * uy;
+    const float w = K.w[k];
+    const float sym = __builtin_fmaf(rho, __builtin_fmaf(hics4 * cu, cu, -hu2), drho);
+    const float asym = rics2 * cu;
+    // geq_k + geq_kb = 2 w sym, geq_k - geq_kb = 2 w asym
+    eS[n] = (g[k] + g[kb]) - (2.0f * w) * sym;
+    eD[n] = (g[k] - g[kb]) - (2.0f * w) * asym;
+    fS[n] = 0.0f;
+    fD[n] = 0.0f;
+    if constexpr (FORCING != D2Q9_FORCING_NONE) {
+      const float cF = (float)CX[k] * fx + (float)CY[k] * fy;
+      fD[n] = (2.0f * w) * (ics2 * cF);
+      if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER)
+        fS[n] = (2.0f * w) * __builtin_fmaf(K.ics4 * cu, cF, -uf);
+    }
+  }
+  float me[9], mf[9];
+  moments_of_pairs(e0, eS, eD, me);
+  if constexpr (FORCING != D2Q9_FORCING_NONE) moments_of_pairs(f0, fS, fD, mf);
+  float gm[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const float sr = K.rates[i];
+    float dm = sr * me[i];
+    if constexpr (FORCING != D2Q9_FORCING_NONE)
+      dm = __builtin_fmaf(-__builtin_fmaf(-0.5f, sr, 1.0f), mf[i], dm);
+    gm[i] = dm * (float)(1.0 / (double)ROWNORM[i]);
+  }
+  // g -= M^T gm, by columns: rest, then per pair the shared and the
+  // opposite-signed part
+  g[0] -= __builtin_fmaf(4.0f, gm[2] - gm[1], gm[0]);
+  const float base13 = __builtin_fmaf(-2.0f, gm[2], gm[0] - gm[1]);
+  const float sym13 = base13 + gm[7], sym24 = base13 - gm[7];
+  const float asym13 = __builtin_fmaf(-2.0f, gm[4], gm[3]);
+  const float asym24 = __builtin_fmaf(-2.0f, gm[6], gm[5]);
+  const float base57 = __builtin_fmaf(2.0f, gm[1], gm[0] + gm[2]);
+  const float sym57 = base57 + gm[8], sym68 = base57 - gm[8];
+  const float p34 = gm[3] + gm[4], p56 = gm[5] + gm[6];
+  const float asym57 = p34 + p56, asym68 = p56 - p34;
+  g[1] -= sym13 + asym13;
+  g[3] -= sym13 - asym13;
+  g[2] -= sym24 + asym24;
+  g[4] -= sym24 - asym24;
+  g[5] -= sym57 + asym57;
+  g[7] -= sym57 - asym57;
+  g[6] -= sym68 + asym68;
+  g[8] -= sym68 - asym68;
+}
+
 // MRT with Guo forcing in place (FP64 operation order of oracle
 // mrt_collide).  The moment map is linear and M w is a constant, so the
 // same code applies to deviations (m - meq is unchanged).
@@ -484,9 +577,12 @@ __device__ __forceinline__ void collide_mrt(T (&g)[9], const Consts<T>& K,
 template <typename T, int MODEL, int FORCING, bool DEV>
 __device__ __forceinline__ void collide(T (&g)[9], const Consts<T>& K, T drho,
                                         T rho, T ux, T uy, T fx, T fy) {
-  if constexpr (MODEL == D2Q9_MRT)
-    collide_mrt<T, FORCING, DEV>(g, K, drho, rho, ux, uy, fx, fy);
-  else
+  if constexpr (MODEL == D2Q9_MRT) {
+    if constexpr (std::is_same<T, float>::value && DEV)
+      collide_mrt_pairs<FORCING>(g, K, drho, rho, ux, uy, fx, fy);
+    else
+      collide_mrt<T, FORCING, DEV>(g, K, drho, rho, ux, uy, fx, fy);
+  } else
     collide_bgk<T, FORCING, DEV>(g, K, drho, rho, ux, uy, fx, fy);
 }
 
