@@ -277,7 +277,9 @@ int d2q9_momentum_exchange(const d2q9_lattice* L, const void* f_pop,
  * atomic counter, then helps the other queues.  waves: 0 lets the compiler
  * allocate registers freely; 6 or 8 selects a build of the kernel whose
  * registers allow that many waves per SIMD (more latency hiding, some
- * register spills).  counters: 16 int32, zero before the first
+ * register spills).  has_general = 0 promises that no item is a general
+ * tile (no lattice holds solid cells) and selects a build without that
+ * path.  counters: 16 int32, zero before the first
  * step; a step of parity p uses counters[8p .. 8p+8) and clears the other
  * half for the next step.  grid must be a multiple of 8.  Entries, items,
  * bounds and counters are device memory.
@@ -297,8 +299,8 @@ int d2q9_batched_step(const d2q9_batch_entry* entries,
                       const d2q9_batch_item* items,
                       const int32_t* queue_bounds, int32_t* counters,
                       int parity, int storage, int tile_i, int grid,
-                      int policy, int chunk, int waves, int write_moments,
-                      void* stream);
+                      int policy, int chunk, int waves, int has_general,
+                      int write_moments, void* stream);
 
 /*
  * Adjoint of one FP64 step (BGK, Guo forcing, per-cell force field;

@@ -122,7 +122,7 @@ SIGNATURES = {
     "d2q9_adjoint_step": (ctypes.c_int, [_LP, ctypes.c_int, _P, _P, _P, _P,
                                          _P, _P]),
     "d2q9_adjoint_init_grad": (ctypes.c_int, [_LP, _P, _P, _P, _P]),
-    "d2q9_batched_step": (ctypes.c_int, [_P, _P, _P, _P] + [ctypes.c_int] * 8
+    "d2q9_batched_step": (ctypes.c_int, [_P, _P, _P, _P] + [ctypes.c_int] * 9
                           + [_P]),
     "d2q9_momentum_exchange": (ctypes.c_int, [_LP, _P, _P, _P, _P,
                                               ctypes.c_int64, _P, ctypes.c_int,

@@ -213,6 +213,8 @@ class BatchedLattices:
                     self.device)
                 self.bounds = torch.from_numpy(bounds).to(self.device)
             self.n_items = int(items.shape[0])
+            self.has_general = int(any(e.tables.n_general > 0
+                                       for e in self.engines))
             self.queue_sizes = np.diff(bounds)
             self._entry_key = None
         self._drop_graphs()
@@ -250,7 +252,7 @@ class BatchedLattices:
             nt.ptr(self.entries), nt.ptr(self.items), nt.ptr(self.bounds),
             nt.ptr(self.counters), int(parity),
             self.engines[0].storage, c.tile_i, c.grid, c.policy, c.chunk,
-            c.waves, int(write_moments), self._sp)
+            c.waves, self.has_general, int(write_moments), self._sp)
 
     def _after_steps(self, k, write_moments):
         flip = k & 1

@@ -22,7 +22,7 @@ namespace batched {
 
 constexpr int PHASE = PHASE_FUSED;
 
-template <int S, int MODEL, int FORCING>
+template <int S, int MODEL, int FORCING, bool GEN>
 __device__ __forceinline__ void run_item(const d2q9_batch_entry& E, int parity, int code,
                                          int write_mom) {
   using C = Codec<S>;
@@ -38,28 +38,32 @@ __device__ __forceinline__ void run_item(const d2q9_batch_entry& E, int parity, 
   } else if (code < L.n_bulk) {
     bulk_tile<S, MODEL, FORCING, PHASE>(L, fs, fd, md, write_mom, ntj, code);
   } else {
-    general_tile<S, MODEL, FORCING, PHASE>(L, fs, fd, ms, md, write_mom, ntj, L.tiles[code]);
+    // a general item in a batch declared without them is skipped
+    if constexpr (GEN)
+      general_tile<S, MODEL, FORCING, PHASE>(L, fs, fd, ms, md, write_mom, ntj, L.tiles[code]);
   }
 }
 
-template <int S>
+template <int S, bool GEN>
 __device__ __forceinline__ void dispatch_item(const d2q9_batch_entry* __restrict__ entries,
                                               const d2q9_batch_item it, int parity,
                                               int write_mom) {
   const d2q9_batch_entry& E = entries[it.lattice];
   switch (E.L.collision * 3 + E.L.forcing) {
-    case 0: run_item<S, D2Q9_BGK, D2Q9_FORCING_NONE>(E, parity, it.code, write_mom); break;
-    case 1: run_item<S, D2Q9_BGK, D2Q9_GUO_LINEAR>(E, parity, it.code, write_mom); break;
-    case 2: run_item<S, D2Q9_BGK, D2Q9_GUO_SECOND_ORDER>(E, parity, it.code, write_mom); break;
-    case 3: run_item<S, D2Q9_MRT, D2Q9_FORCING_NONE>(E, parity, it.code, write_mom); break;
-    case 4: run_item<S, D2Q9_MRT, D2Q9_GUO_LINEAR>(E, parity, it.code, write_mom); break;
-    default: run_item<S, D2Q9_MRT, D2Q9_GUO_SECOND_ORDER>(E, parity, it.code, write_mom); break;
+    case 0: run_item<S, D2Q9_BGK, D2Q9_FORCING_NONE, GEN>(E, parity, it.code, write_mom); break;
+    case 1: run_item<S, D2Q9_BGK, D2Q9_GUO_LINEAR, GEN>(E, parity, it.code, write_mom); break;
+    case 2: run_item<S, D2Q9_BGK, D2Q9_GUO_SECOND_ORDER, GEN>(E, parity, it.code, write_mom); break;
+    case 3: run_item<S, D2Q9_MRT, D2Q9_FORCING_NONE, GEN>(E, parity, it.code, write_mom); break;
+    case 4: run_item<S, D2Q9_MRT, D2Q9_GUO_LINEAR, GEN>(E, parity, it.code, write_mom); break;
+    default: run_item<S, D2Q9_MRT, D2Q9_GUO_SECOND_ORDER, GEN>(E, parity, it.code, write_mom); break;
   }
 }
 
 // W: waves per SIMD the register allocation must allow (1: no limit
 // beyond the compiler's own choice).
-template <int S, int W>
+// GEN false: no lattice of the batch has general tiles (solid cells), and
+// the general-tile path is left out of the kernel (fewer registers).
+template <int S, int W, bool GEN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_batched(const d2q9_batch_entry* __restrict__ entries,
                                                  const d2q9_batch_item* __restrict__ items,
                                                  const int32_t* __restrict__ qb,
@@ -75,7 +79,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
   if (blockIdx.x == 0 && tid < 8) counters[(1 - parity) * 8 + tid] = 0;
   if (policy == 0) {
     for (int it = qb[xcd] + r; it < qb[xcd + 1]; it += per)
-      dispatch_item<S>(entries, items[it], parity, write_mom);
+      dispatch_item<S, GEN>(entries, items[it], parity, write_mom);
     return;
   }
   for (int s = 0; s < 8; ++s) {
@@ -89,7 +93,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       if (k0 >= n) break;
       const int k1 = min(k0 + chunk, n);
       for (int k = k0; k < k1; ++k)
-        dispatch_item<S>(entries, items[lo + k], parity, write_mom);
+        dispatch_item<S, GEN>(entries, items[lo + k], parity, write_mom);
     }
   }
 }
@@ -100,32 +104,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
 template <int S>
 hipError_t launch(const d2q9_batch_entry* entries, const d2q9_batch_item* items,
                   const int32_t* qb, int32_t* counters, int parity, int tile_i,
-                  int grid, int policy, int chunk, int waves, int write_mom,
-                  hipStream_t s) {
+                  int grid, int policy, int chunk, int waves, int has_general,
+                  int write_mom, hipStream_t s) {
   const dim3 g(grid), b(64, tile_i);
-  switch (waves) {
-    case 0:
-      hipLaunchKernelGGL((k_batched<S, 1>), g, b, 0, s, entries, items, qb, counters,
-                         parity, policy, chunk, write_mom);
-      break;
-    case 6:
-      hipLaunchKernelGGL((k_batched<S, 6>), g, b, 0, s, entries, items, qb, counters,
-                         parity, policy, chunk, write_mom);
-      break;
-    case 8:
-      hipLaunchKernelGGL((k_batched<S, 8>), g, b, 0, s, entries, items, qb, counters,
-                         parity, policy, chunk, write_mom);
-      break;
+#define D2Q9_BLAUNCH(WV, GN)                                                            \
+  hipLaunchKernelGGL((k_batched<S, WV, GN>), g, b, 0, s, entries, items, qb, counters, \
+                     parity, policy, chunk, write_mom)
+  switch (waves * 2 + (has_general ? 1 : 0)) {
+    case 0: D2Q9_BLAUNCH(1, false); break;
+    case 1: D2Q9_BLAUNCH(1, true); break;
+    case 12: D2Q9_BLAUNCH(6, false); break;
+    case 13: D2Q9_BLAUNCH(6, true); break;
+    case 16: D2Q9_BLAUNCH(8, false); break;
+    case 17: D2Q9_BLAUNCH(8, true); break;
     default:
       return hipErrorInvalidValue;
   }
+#undef D2Q9_BLAUNCH
   return hipGetLastError();
 }
 
 #define D2Q9_BATCHED_DECLARE(SV)                                                        \
   extern template hipError_t launch<SV>(const d2q9_batch_entry*, const d2q9_batch_item*, \
                                         const int32_t*, int32_t*, int, int, int, int,   \
-                                        int, int, int, hipStream_t);
+                                        int, int, int, int, hipStream_t);
 D2Q9_BATCHED_DECLARE(D2Q9_STORE_F64)
 D2Q9_BATCHED_DECLARE(D2Q9_STORE_F32)
 #undef D2Q9_BATCHED_DECLARE

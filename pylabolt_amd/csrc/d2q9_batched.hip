@@ -10,8 +10,8 @@ extern "C" int d2q9_batched_step(const d2q9_batch_entry* entries,
                                  const d2q9_batch_item* items,
                                  const int32_t* queue_bounds, int32_t* counters,
                                  int parity, int storage, int tile_i, int grid,
-                                 int policy, int chunk, int waves, int write_moments,
-                                 void* stream) {
+                                 int policy, int chunk, int waves, int has_general,
+                                 int write_moments, void* stream) {
   if (!entries || !items || !queue_bounds || !counters) {
     set_error("batched step: null buffer");
     return D2Q9_E_ARG;
@@ -30,11 +30,11 @@ extern "C" int d2q9_batched_step(const d2q9_batch_entry* entries,
   switch (storage) {
     case D2Q9_STORE_F64:
       e = batched::launch<D2Q9_STORE_F64>(entries, items, queue_bounds, counters, parity,
-                                          tile_i, grid, policy, chunk, waves, write_moments, s);
+                                          tile_i, grid, policy, chunk, waves, has_general, write_moments, s);
       break;
     case D2Q9_STORE_F32:
       e = batched::launch<D2Q9_STORE_F32>(entries, items, queue_bounds, counters, parity,
-                                          tile_i, grid, policy, chunk, waves, write_moments, s);
+                                          tile_i, grid, policy, chunk, waves, has_general, write_moments, s);
       break;
     default:
       set_error("batched step: FP64 or FP32 storage only");

@@ -18,13 +18,14 @@ from tests.helpers import max_rel, oracle_case
 pytestmark = pytest.mark.gpu
 
 
-def _mixed_sims(precision="single"):
+def _mixed_sims(precision="single", obstacle=True):
     sims = cases.batch_cases(n=6, seed=5, nx_range=(32, 96),
                              ny_range=(64, 320), precision=precision)
-    cyl = cases.cylinder(48, 140, diameter=12, xc=16)
-    if precision == "single":
-        cyl.control_dict["precision"] = "single"
-    sims.append(cyl)
+    if obstacle:
+        cyl = cases.cylinder(48, 140, diameter=12, xc=16)
+        if precision == "single":
+            cyl.control_dict["precision"] = "single"
+        sims.append(cyl)
     return sims
 
 
@@ -46,27 +47,31 @@ def _assert_same(a, b, label):
 
 
 CONFIGS = [
-    (4, "static", None, 1, 0),
-    (4, "dynamic", None, 1, 0),
-    (4, "dynamic", 64, 4, 6),
-    (2, "dynamic", 24, 3, 8),
-    (1, "static", 8, 1, 6),
-    (2, "static", 4096, 1, 8),
+    (4, "static", None, 1, 0, True),
+    (4, "dynamic", None, 1, 0, True),
+    (4, "dynamic", 64, 4, 6, True),
+    (2, "dynamic", 24, 3, 8, True),
+    (1, "static", 8, 1, 6, True),
+    (2, "static", 4096, 1, 8, True),
+    # no obstacle: the build without the general-tile path
+    (4, "dynamic", None, 4, 0, False),
+    (1, "static", 16, 1, 6, False),
 ]
 
 
-@pytest.mark.parametrize("tile_i,policy,grid,chunk,waves", CONFIGS)
+@pytest.mark.parametrize("tile_i,policy,grid,chunk,waves,obstacle", CONFIGS)
 def test_batched_fp32_bitwise_equals_per_lattice(tile_i, policy, grid, chunk,
-                                                 waves):
+                                                 waves, obstacle):
     from pylabolt_amd.batch import BatchConfig, BatchedLattices
     cfg = BatchConfig.from_dict({"tile_i": tile_i, "policy": policy,
                                  "grid": grid, "chunk": chunk,
                                  "waves": waves})
-    sims = _mixed_sims()
+    sims = _mixed_sims(obstacle=obstacle)
     print(f"[batched] f32 {cfg} lattices "
           f"{[tuple(s.mesh_dict['grid']) for s in sims]}", flush=True)
     batch = BatchedLattices(sims, storage="f32", config=cfg, graph_steps=8)
-    solo = [_solo(s, "f32") for s in _mixed_sims()]
+    assert batch.has_general == int(obstacle)
+    solo = [_solo(s, "f32") for s in _mixed_sims(obstacle=obstacle)]
     done = 0
     for n in (1, 1, 19, 17):
         batch.advance(n)
@@ -158,6 +163,6 @@ def test_batched_rejects_bad_arguments():
         p, st, ti, g, pol, ch, wv = args
         rc = lib.d2q9_batched_step(nt.ptr(b.entries), nt.ptr(b.items),
                                    nt.ptr(b.bounds), nt.ptr(b.counters),
-                                   p, st, ti, g, pol, ch, wv, 0, b._sp)
+                                   p, st, ti, g, pol, ch, wv, 1, 0, b._sp)
         assert rc != 0
     b.close()
