@@ -158,14 +158,20 @@ typedef struct d2q9_lattice {
   const d2q9_bc_node* bc_nodes;
   const d2q9_bc_elem* bc_elems;
   const void* force_field;      /* [2][plane] when force_source == FIELD */
-  /* Zero-gradient rows swept inside bulk tiles (null: none).  For bulk
-   * tile p and row r, words [((p * tile_i + r) * 2 + s) * cpl, + cpl) mark
-   * (one bit per cell, as bulk_masks) the cells whose only boundary rule
-   * is zero-gradient from a wall across y: s = 0 takes directions 2,5,6
-   * from the cell at j + 1, s = 1 takes 4,7,8 from the cell at j - 1, both
-   * as pulled before any boundary rule.  Those cells are not masked in
-   * bulk_masks and not in the fix-up list. */
-  const uint64_t* zg_masks;
+  /* Wall cells swept inside bulk tiles (null: none).  wall_index[p] is the
+   * slot of bulk tile p in wall_masks, or -1 when the tile has no such
+   * cell.  For slot t and row r, words [((t * tile_i + r) * 6 + s) * cpl,
+   * + cpl) mark (one bit per cell, as bulk_masks) the cells whose only
+   * boundary rules are those of one straight wall:
+   *   s = 0: zero-gradient, directions 2,5,6 from the cell at j + 1,
+   *   s = 1: zero-gradient, directions 4,7,8 from the cell at j - 1,
+   *          (both as that cell pulled them, before any boundary rule);
+   *   s = 2..5: bounce-back on the wall of normal (0,1), (0,-1), (1,0),
+   *          (-1,0): directions 2,5,6 / 4,7,8 / 1,5,8 / 3,6,7 take the
+   *          cell's own post-collision value of the opposite direction.
+   * Those cells are not masked in bulk_masks and not in the fix-up list. */
+  const int32_t* wall_index;
+  const uint64_t* wall_masks;
 } d2q9_lattice;
 
 /* Populations f = feq(rho0, u0) on fluid cells of both buffers, the encoded

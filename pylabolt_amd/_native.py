@@ -72,7 +72,7 @@ class Lattice(ctypes.Structure):
         ("bulk_masks", ctypes.c_void_p),
         ("bc_slot", ctypes.c_void_p), ("bc_nodes", ctypes.c_void_p),
         ("bc_elems", ctypes.c_void_p), ("force_field", ctypes.c_void_p),
-        ("zg_masks", ctypes.c_void_p),
+        ("wall_index", ctypes.c_void_p), ("wall_masks", ctypes.c_void_p),
     ]
 
 
@@ -134,7 +134,7 @@ ABI_FIELDS = ["plane", "cpl", "pop_len", "storage", "tile_j", "omega", "w",
               "gravity", "rates", "dev_scale", "cf", "flags", "tiles",
               "n_bulk", "fix_cells", "n_fix", "bulk_masks", "bc_slot",
               "bc_nodes",
-              "bc_elems", "force_field", "zg_masks"]
+              "bc_elems", "force_field", "wall_index", "wall_masks"]
 
 
 def abi_mismatches(handle):

@@ -248,6 +248,8 @@ int validate(const d2q9_lattice* L) {
     const int cpl = L->storage == D2Q9_STORE_F64 ? 1 : (L->storage == D2Q9_STORE_F32 ? 2 : CPL16);
     const int B = 64 * cpl;
     if (L->cpl != cpl) return fail(D2Q9_E_ARG, "cpl does not match the storage");
+    if (L->wall_masks && !L->wall_index)
+      return fail(D2Q9_E_ARG, "wall_masks needs wall_index");
     if (L->storage == D2Q9_STORE_F32 || L->storage == D2Q9_STORE_BF16DEV)
       for (int k = 0; k < 9; ++k)
         if (L->dev_scale[k] != 1.0 || L->cf.scale[k] != 1.0f)
@@ -579,11 +581,11 @@ int d2q9_abi_layout(int64_t* out, int n) {
       (int64_t)offsetof(d2q9_lattice, bc_slot),
       (int64_t)offsetof(d2q9_lattice, bc_nodes), (int64_t)offsetof(d2q9_lattice, bc_elems),
       (int64_t)offsetof(d2q9_lattice, force_field),
-      (int64_t)offsetof(d2q9_lattice, zg_masks),
+      (int64_t)offsetof(d2q9_lattice, wall_index), (int64_t)offsetof(d2q9_lattice, wall_masks),
       (int64_t)sizeof(d2q9_batch_entry), (int64_t)offsetof(d2q9_batch_entry, f),
       (int64_t)offsetof(d2q9_batch_entry, mom), (int64_t)sizeof(d2q9_batch_item)};
   const int m = (int)(sizeof(v) / sizeof(v[0]));
-  if (!out || n < m) return fail(D2Q9_E_ARG, "abi layout needs 29 slots");
+  if (!out || n < m) return fail(D2Q9_E_ARG, "abi layout needs 30 slots");
   for (int q = 0; q < m; ++q) out[q] = v[q];
   return m;
 }

@@ -224,37 +224,48 @@ __device__ __forceinline__ void store_block(
   }
 }
 
-// Zero-gradient wall rows inside a bulk tile: a marked cell takes the
+// Straight-wall cells inside a bulk tile (masks per row and side, uniform
+// over the wave).  Zero-gradient (sides 0, 1): a marked cell takes the
 // unknown directions (2,5,6 from the cell at j + 1, or 4,7,8 from the cell
 // at j - 1) as that cell pulled them, before any boundary rule -- the same
 // values the rule reads (apply_bc, zero-gradient branch) when the source
-// cell has no solid neighbour and no periodic wrap, which the host checks.
-// The source is the next cell of the lane or, at a lane edge, of the
-// neighbouring lane (one shuffle).  The masks are uniform over the wave.
-template <int S>
-__device__ __forceinline__ void zero_gradient_rows(const d2q9_lattice& L,
-                                                   typename Codec<S>::T (&g)[Codec<S>::CPL][9],
-                                                   int p, int lane) {
+// cell has no solid neighbour and no periodic wrap, which the host checks;
+// the source is the next cell of the lane or, at a lane edge, of the
+// neighbouring lane (one shuffle).  Bounce-back (sides 2..5): the unknown
+// directions take the cell's own post-collision value of the opposite
+// direction (fluid_boundary_kernels.py:5-25), one load each, lines the
+// tile's neighbours have just read.
+template <int S, int PHASE>
+__device__ __forceinline__ void wall_rows(const d2q9_lattice& L,
+                                          const Consts<typename Codec<S>::T>& K,
+                                          const typename Codec<S>::raw* __restrict__ fsrc,
+                                          typename Codec<S>::T (&g)[Codec<S>::CPL][9], int p,
+                                          int lane, int i, int j0) {
   using T = typename Codec<S>::T;
   constexpr int CPL = Codec<S>::CPL;
-  const uint64_t* zrow = L.zg_masks + ((int64_t)p * blockDim.y + threadIdx.y) * 2 * CPL;
-  uint32_t up = 0, down = 0;
-  bool any_up = false, any_down = false;
+  const int slot = L.wall_index[p];
+  if (slot < 0) return;
+  const uint64_t* zrow = L.wall_masks + ((int64_t)slot * blockDim.y + threadIdx.y) * 6 * CPL;
+  bool any[6];
 #pragma unroll
-  for (int w = 0; w < CPL; ++w) {
-    any_up |= zrow[w] != 0;
-    any_down |= zrow[CPL + w] != 0;
-  }
-  if (!any_up && !any_down) return;
+  for (int s = 0; s < 6; ++s) {
+    any[s] = false;
 #pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    const int o = lane * CPL + q;
-    up |= (uint32_t)((zrow[o >> 6] >> (o & 63)) & 1ull) << q;
-    down |= (uint32_t)((zrow[CPL + (o >> 6)] >> (o & 63)) & 1ull) << q;
+    for (int w = 0; w < CPL; ++w) any[s] |= zrow[s * CPL + w] != 0;
   }
+  auto lane_bits = [&](int s) {
+    uint32_t b = 0;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      const int o = lane * CPL + q;
+      b |= (uint32_t)((zrow[s * CPL + (o >> 6)] >> (o & 63)) & 1ull) << q;
+    }
+    return b;
+  };
   constexpr int DU[3] = {2, 5, 6};
   constexpr int DD[3] = {4, 7, 8};
-  if (any_up) {
+  if (any[0]) {
+    const uint32_t up = lane_bits(0);
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
       const int d = DU[n];
@@ -266,7 +277,8 @@ __device__ __forceinline__ void zero_gradient_rows(const d2q9_lattice& L,
       }
     }
   }
-  if (any_down) {
+  if (any[1]) {
+    const uint32_t down = lane_bits(1);
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
       const int d = DD[n];
@@ -275,6 +287,21 @@ __device__ __forceinline__ void zero_gradient_rows(const d2q9_lattice& L,
       for (int q = CPL - 1; q >= 0; --q) {
         const T src = (q > 0) ? g[q - 1][d] : prv;
         if ((down >> q) & 1) g[q][d] = src;
+      }
+    }
+  }
+  constexpr int BD[4][3] = {{2, 5, 6}, {4, 7, 8}, {1, 5, 8}, {3, 6, 7}};
+#pragma unroll
+  for (int s = 2; s < 6; ++s) {
+    if (!any[s]) continue;
+    const uint32_t bits = lane_bits(s);
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      if (!((bits >> q) & 1)) continue;
+#pragma unroll
+      for (int n = 0; n < 3; ++n) {
+        const int d = BD[s - 2][n];
+        g[q][d] = ld<S>(L, K, fsrc, i, j0 + q, INV[d]);
       }
     }
   }
@@ -326,7 +353,8 @@ __device__ __forceinline__ BulkPos bulk_pos(const d2q9_lattice& L, int ntj, int 
 template <int S, int MODEL, int FORCING, int PHASE>
 __device__ __forceinline__ void bulk_finish(
     const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K, const RawBlock<S>& rb,
-    const BulkPos& bp, typename Codec<S>::raw* __restrict__ fdst,
+    const BulkPos& bp, const typename Codec<S>::raw* __restrict__ fsrc,
+    typename Codec<S>::raw* __restrict__ fdst,
     typename Codec<S>::T* __restrict__ mdst, int write_mom, int p) {
   using C = Codec<S>;
   using T = typename C::T;
@@ -336,7 +364,7 @@ __device__ __forceinline__ void bulk_finish(
   T g[CPL][9];
   gather_finish<S, PHASE>(K, rb, lane, g);
   if constexpr (PHASE != PHASE_COLLIDE) {
-    if (L.zg_masks) zero_gradient_rows<S>(L, g, p, lane);
+    if (L.wall_masks) wall_rows<S, PHASE>(L, K, fsrc, g, p, lane, bp.i, bp.j0);
   }
   typename C::raw out[9][CPL];
 #pragma unroll
@@ -372,7 +400,7 @@ __device__ __forceinline__ void bulk_tile(
   const Consts<T> K = load_consts<T>(L);
   RawBlock<S> rb;
   gather_issue<S, PHASE>(L, fsrc, bp.i, bp.tj, (int)threadIdx.x, bp.j0, rb);
-  bulk_finish<S, MODEL, FORCING, PHASE>(L, K, rb, bp, fdst, mdst, write_mom, p);
+  bulk_finish<S, MODEL, FORCING, PHASE>(L, K, rb, bp, fsrc, fdst, mdst, write_mom, p);
 }
 
 // Fix-up cells [first, first + workgroup size) of the lattice's list, one

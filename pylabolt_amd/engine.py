@@ -151,20 +151,27 @@ class CaseTables:
             self.bc_nodes[n, :9] = rules[key]
             self.bc_slot_pad[key] = n
         self.n_bc_nodes = len(keys)
-        # zero-gradient wall cells across y whose only rules are that wall's
-        # (side 0: normal (0, 1), directions 2,5,6 from j + 1; side 1:
-        # normal (0, -1), directions 4,7,8 from j - 1)
-        self.zg_side_pad = np.full((nxp, nyp), -1, dtype=np.int8)
+        # straight-wall cells whose only rules are one wall's: the bulk
+        # sweep can treat them itself (sides of the device's wall masks:
+        # 0/1 zero-gradient from j + 1 / j - 1, 2..5 bounce-back on the
+        # walls of normal (0,1), (0,-1), (1,0), (-1,0))
+        self.wall_side_pad = np.full((nxp, nyp), -1, dtype=np.int8)
+        zg_sides = {((0, 1), (2, 5, 6)): 0, ((0, -1), (4, 7, 8)): 1}
+        bb_sides = {((0, 1), (2, 5, 6)): 2, ((0, -1), (4, 7, 8)): 3,
+                    ((1, 0), (1, 5, 8)): 4, ((-1, 0), (3, 6, 7)): 5}
         for (i, j), r in rules.items():
-            dirs = [k for k in range(9) if r[k] & 15 != nt.RULE_STREAM]
-            if not dirs or any(r[k] & 15 != nt.RULE_ZG for k in dirs):
-                continue
+            dirs = tuple(k for k in range(9) if r[k] & 15 != nt.RULE_STREAM)
+            kinds = {r[k] & 15 for k in dirs}
             normals = {tuple(int(v) for v in self.elems[r[k] >> 4][1]["normal"])
                        for k in dirs}
-            if normals == {(0, 1)} and sorted(dirs) == [2, 5, 6]:
-                self.zg_side_pad[i, j] = 0
-            elif normals == {(0, -1)} and sorted(dirs) == [4, 7, 8]:
-                self.zg_side_pad[i, j] = 1
+            if len(kinds) != 1 or len(normals) != 1:
+                continue
+            key = (normals.pop(), dirs)
+            kind = kinds.pop()
+            if kind == nt.RULE_ZG and key in zg_sides:
+                self.wall_side_pad[i, j] = zg_sides[key]
+            elif kind == nt.RULE_BB and key in bb_sides:
+                self.wall_side_pad[i, j] = bb_sides[key]
         self.set_tile(tile_j, tile_i)
 
     def set_tile(self, tile_j, tile_i):
@@ -193,18 +200,19 @@ class CaseTables:
             special[:, 0] = True
             special[:, -1] = True
         special &= fluid
-        # zero-gradient cells the bulk sweep handles itself (copy from the
-        # neighbour in y): plain boundary cells whose source is a plain pull
-        # of the same block, not itself such a cell
-        side = self.zg_side_pad[1:-1, 1:-1]
+        # straight-wall cells the bulk sweep handles itself: plain boundary
+        # cells (no solid neighbour, no stored moment, no periodic edge);
+        # for zero-gradient, the source cell must be a plain pull of the
+        # same block and not itself such a cell
+        side = self.wall_side_pad[1:-1, 1:-1]
         edge = np.zeros_like(fluid)
         if self.x_periodic:
             edge[0, :] = edge[-1, :] = True
         if self.y_periodic:
             edge[:, 0] = edge[:, -1] = True
         plain_bc = (inner == (nt.CELL_FLUID | nt.CELL_BC)) & ~edge
-        cand = plain_bc & (side >= 0)
-        zg = np.zeros_like(fluid)
+        cand = plain_bc & (side >= 0) & (side <= 1)
+        zg = plain_bc & (side >= 2)
         for s_, step in ((0, 1), (1, -1)):
             ci, cj = np.nonzero(cand & (side == s_))
             sj = cj + step
@@ -216,7 +224,7 @@ class CaseTables:
                   ~edge[ci, sj] & ~cand[ci, sj] &
                   (cj // self.tile_j == sj // self.tile_j))
             zg[ci[ok], cj[ok]] = True
-        self.zg_inline = zg
+        self.wall_inline = zg
         special &= ~zg
         ntj = -(-ny // self.tile_j)
         nti = -(-nx // self.tile_i)
@@ -247,9 +255,9 @@ class CaseTables:
         self.bulk_masks = np.ascontiguousarray(m[bulk_ids]).reshape(-1)
         if self.bulk_masks.size == 0:
             self.bulk_masks = np.zeros(1, dtype=np.uint64)
-        # zero-gradient bits per (bulk tile, row, side, word)
+        # wall bits per (bulk tile, row, side, word)
         zm = []
-        for s_ in (0, 1):
+        for s_ in range(6):
             zpad = np.zeros((nti * self.tile_i, ntj * self.tile_j), dtype=bool)
             zpad[:nx, :ny] = zg & (side == s_)
             zb = zpad.reshape(nti, self.tile_i, ntj, words, 64)
@@ -257,10 +265,13 @@ class CaseTables:
             zm.append(w_.transpose(0, 2, 1, 3).reshape(nti * ntj, self.tile_i,
                                                        words))
         zmask = np.stack(zm, axis=2)[bulk_ids]   # [bulk, row, side, word]
-        self.n_zg = int((zg & in_bulk_cells(tiles, self.tile_i, self.tile_j,
-                                            nx, ny)).sum())
-        self.zg_masks = (np.ascontiguousarray(zmask).reshape(-1)
-                         if self.n_zg else None)
+        self.n_wall = int((zg & in_bulk_cells(tiles, self.tile_i, self.tile_j,
+                                              nx, ny)).sum())
+        # compact: only the bulk tiles holding wall cells get a slot
+        has = zmask.reshape(zmask.shape[0], -1).any(axis=1)
+        self.wall_index = np.where(has, np.cumsum(has) - 1, -1).astype(np.int32)
+        self.wall_masks = (np.ascontiguousarray(zmask[has]).reshape(-1)
+                           if self.n_wall else None)
         ii, jj = np.nonzero(special & in_bulk)
         pitch = int(math.ceil((ny + 2) / ALIGN) * ALIGN)
         self.fix_cells = (ORIGIN + (ii + 1) * pitch + (jj + 1)).astype(np.int32)
@@ -451,13 +462,17 @@ class D2Q9Engine:
         L.fix_cells = self.fix_cells.data_ptr()
         L.n_fix = self.tables.n_fix
         L.bulk_masks = self.bulk_masks.data_ptr()
-        self.zg_masks = None
-        if self.tables.zg_masks is not None:
+        self.wall_masks = self.wall_index = None
+        if self.tables.wall_masks is not None:
             with torch.cuda.stream(self.stream):
-                self.zg_masks = torch.from_numpy(
-                    self.tables.zg_masks.view(np.int64)).to(self.device)
-        L.zg_masks = (self.zg_masks.data_ptr() if self.zg_masks is not None
-                      else None)
+                self.wall_masks = torch.from_numpy(
+                    self.tables.wall_masks.view(np.int64)).to(self.device)
+                self.wall_index = torch.from_numpy(
+                    self.tables.wall_index).to(self.device)
+        L.wall_masks = (self.wall_masks.data_ptr()
+                        if self.wall_masks is not None else None)
+        L.wall_index = (self.wall_index.data_ptr()
+                        if self.wall_index is not None else None)
 
     def set_tile_rows(self, tile_i):
         """Re-tile with ``tile_i`` rows per tile (workgroup 64 x tile_i);

@@ -84,40 +84,57 @@ def test_cavity_corner_rules_last_element_wins():
     assert kinds[1] == nt.RULE_BB and kinds[5] == nt.RULE_BB
 
 
+@pytest.mark.parametrize("name", ["cylinder", "cavity", "channel"])
 @pytest.mark.parametrize("B", [64, 128, 256])
-def test_zero_gradient_rows_swept_inline(B):
-    """Cylinder: the zero-gradient top/bottom cells away from the corners
-    and from the pressure columns' stored-moment neighbours are swept
-    inside their bulk tiles (zg bits), the rest of the special cells go to
-    the fix-up list; together they are all special cells of bulk tiles."""
-    st, t, nx, ny = _tables("cylinder", B)
+def test_wall_rows_swept_inline(name, B):
+    """Straight-wall cells whose only rules are one wall's (zero-gradient
+    rows of the cylinder, bounce-back walls of the cavity and channel) are
+    swept inside their bulk tiles (wall bits), the other special cells go
+    to the fix-up list; together they are all special cells of bulk tiles.
+    Corners, the moving lid, stored-moment and periodic-edge cells stay in
+    the list."""
+    st, t, nx, ny = _tables(name, B)
     pitch, _ = E.geometry(nx, ny)
     fix = set(int(c) for c in t.fix_cells[:t.n_fix])
     inner = t.flags_pad[1:-1, 1:-1]
     fluid = (inner & 1) != 0
     special = (inner != 1) & fluid
+    if t.x_periodic:
+        special[0, :] |= fluid[0, :]
+        special[-1, :] |= fluid[-1, :]
     in_bulk = E.in_bulk_cells(t.tile_class.reshape(t.nti, t.ntj) == 0,
                               t.tile_i, t.tile_j, nx, ny)
-    zg = set()
-    zm = t.zg_masks.reshape(t.n_bulk, 4, 2, B // 64)
+    wall = set()
+    zm = t.wall_masks.reshape(-1, 4, 6, B // 64)
+    assert t.wall_index.size == t.n_bulk
+    assert zm.shape[0] == int((t.wall_index >= 0).sum())
     for p in range(t.n_bulk):
         tid = int(t.tile_list[p])
         tj, ti = tid % t.ntj, tid // t.ntj
+        slot = int(t.wall_index[p])
+        if slot < 0:
+            continue
         for r in range(4):
-            for s in range(2):
+            for s in range(6):
                 for w in range(B // 64):
-                    word = int(zm[p, r, s, w])
+                    word = int(zm[slot, r, s, w])
                     for b in range(64):
                         if (word >> b) & 1:
                             i = ti * 4 + r          # interior index
                             j = tj * B + 64 * w + b
-                            assert t.zg_side_pad[i + 1, j + 1] == s
-                            # bottom row takes from above, top from below
-                            assert j == (0 if s == 0 else ny - 1)
-                            zg.add(E.ORIGIN + (i + 1) * pitch + (j + 1))
+                            assert t.wall_side_pad[i + 1, j + 1] == s
+                            # the side matches the wall the cell lies on
+                            edge = {0: j == 0, 1: j == ny - 1, 2: j == 0,
+                                    3: j == ny - 1, 4: i == 0,
+                                    5: i == nx - 1}[s]
+                            assert edge, (s, i, j)
+                            wall.add(E.ORIGIN + (i + 1) * pitch + (j + 1))
     want = set(E.ORIGIN + (i + 1) * pitch + (j + 1)
                for i, j in zip(*np.nonzero(special & in_bulk)))
-    assert not (zg & fix)
-    assert zg | fix == want
-    # the pressure columns' neighbours (stored moments) and corners stay
-    assert len(zg) == t.n_zg and 0 < len(zg) <= 2 * (nx - 4)
+    assert not (wall & fix)
+    assert wall | fix == want
+    assert len(wall) == t.n_wall and len(wall) > 0
+    if name == "cavity":
+        # the lid (moving wall) and the corners stay in the list
+        lid = set(E.ORIGIN + (i + 1) * pitch + ny for i in range(nx))
+        assert lid <= fix
