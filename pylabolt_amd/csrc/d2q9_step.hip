@@ -74,6 +74,30 @@ __global__ __launch_bounds__(256) void k_general_fix(
   general_tile<S, MODEL, FORCING, PHASE>(L, fsrc, fdst, msrc, mdst, write_mom, ntj, t);
 }
 
+// One kernel for the whole step: workgroups [0, ngen) sweep the general
+// tiles, the next nfixb the fix-up chunks, the rest the bulk tiles.  Used
+// for FP32 storage, where it beats the two overlapped kernels (no
+// cross-stream join; the FP32 bulk path has registers to spare).
+template <int S, int MODEL, int FORCING, int PHASE>
+__global__ __launch_bounds__(256) void k_step_all(
+    const d2q9_lattice L, const typename Codec<S>::raw* __restrict__ fsrc,
+    typename Codec<S>::raw* __restrict__ fdst,
+    const typename Codec<S>::T* __restrict__ msrc,
+    typename Codec<S>::T* __restrict__ mdst, int write_mom, int ntj, int nfixb) {
+  const int b = (int)blockIdx.x;
+  const int ngen = L.n_general;
+  if (b < ngen) {
+    general_tile<S, MODEL, FORCING, PHASE>(L, fsrc, fdst, msrc, mdst, write_mom, ntj,
+                                           L.tiles[L.n_bulk + b]);
+  } else if (b < ngen + nfixb) {
+    fix_chunk<S, MODEL, FORCING, PHASE>(L, fsrc, fdst, msrc, mdst, write_mom,
+                                        (b - ngen) * 64 * (int)blockDim.y);
+  } else {
+    bulk_tile<S, MODEL, FORCING, PHASE>(L, fsrc, fdst, mdst, write_mom, ntj,
+                                        xcd_remap(b - ngen - nfixb, L.n_bulk));
+  }
+}
+
 // f = feq(rho0, u0) on fluid cells, the encoded zero on every other padded
 // cell, in both buffers.
 template <int S>
@@ -325,6 +349,14 @@ int launch_sweep(const d2q9_lattice* L, const void* fsrc, void* fdst,
   if (L->n_bulk > 0 && !L->bulk_masks) return fail(D2Q9_E_ARG, "null bulk_masks");
   const int wg = 64 * L->tile_i;
   const int nfixb = (L->n_fix + wg - 1) / wg;
+  if constexpr (S == D2Q9_STORE_F32 && PHASE == PHASE_FUSED) {
+    if (L->n_bulk > 0) {
+      const unsigned nb = (unsigned)(L->n_general + nfixb + L->n_bulk);
+      hipLaunchKernelGGL((k_step_all<S, MODEL, FORCING, PHASE>), dim3(nb), block, 0, s, *L,
+                         fs, fd, ms, md, write_mom, ntj, nfixb);
+      return check_hip(hipGetLastError(), "k_step_all launch");
+    }
+  }
   hipStream_t aux = s;
   Aux* a = nullptr;
   if (L->n_bulk > 0 && L->n_general + nfixb > 0) {
