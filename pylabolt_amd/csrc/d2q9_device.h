@@ -133,12 +133,38 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
 // bit, while the rounding of a slowly changing field is decorrelated from
 // step to step.  The cell's key is hashed once per cell (cell_key); per
 // value one 24-bit multiply (full rate) spreads the low 24 bits of
-// value ^ key into bits 16..31, of which the caller keeps the low 13 (half)
-// or 16 (bfloat16).
+// value ^ key into the upper bits of the product, which the conversions
+// below use: bits 16..31 for bfloat16, bits 19..31 for half.
 __device__ __forceinline__ uint32_t cell_key(uint32_t salt) { return mix32(salt); }
 __device__ __forceinline__ uint32_t round_bits(uint32_t b, int k, uint32_t key) {
   const uint32_t x = b ^ (key + (uint32_t)k * 0x632be5abu);
-  return __umul24(x, 0x9e3779u) >> 16;
+  return __umul24(x, 0x9e3779u);
+}
+
+// gfx950's stochastic-rounding conversions.  v_cvt_sr_bf16_f32 adds bits
+// 16..31 of the random operand below the 16 kept bits of the float and
+// truncates; v_cvt_sr_f16_f32 adds bits 19..31 below half's 10 kept
+// mantissa bits over half's normal range and rounds stochastically on the
+// subnormal grid below it.  Each writes one half of a 32-bit register
+// (word_sel), so a pair of values packs with no extra instruction.
+typedef _Float16 half2_sr __attribute__((ext_vector_type(2)));
+typedef __bf16 bf162_sr __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t sr_half2(float a0, uint32_t r0, float a1, uint32_t r1) {
+  half2_sr z = {};
+  z = __builtin_amdgcn_cvt_sr_f16_f32(z, a0, r0, false);
+  z = __builtin_amdgcn_cvt_sr_f16_f32(z, a1, r1, true);
+  uint32_t w;
+  __builtin_memcpy(&w, &z, 4);
+  return w;
+}
+__device__ __forceinline__ uint32_t sr_bf162(float a0, uint32_t r0, float a1, uint32_t r1) {
+  bf162_sr z = {};
+  z = __builtin_amdgcn_cvt_sr_bf16_f32(z, a0, r0, false);
+  z = __builtin_amdgcn_cvt_sr_bf16_f32(z, a1, r1, true);
+  uint32_t w;
+  __builtin_memcpy(&w, &z, 4);
+  return w;
 }
 
 template <int S> struct Codec;
@@ -182,13 +208,22 @@ template <> struct Codec<D2Q9_STORE_F16DEV> {
   static __device__ __forceinline__ raw enc(T v, int k, const Consts<T>& K) {
     return __float2half_rn(v * K.scale[k]);
   }
-  // Stochastic rounding to half: random bits added below the 10 kept
-  // mantissa bits, then truncation.  The truncated float is exact in half
-  // over half's normal range; below it the conversion rounds to nearest.
+  // Stochastic rounding to half of the scaled deviation.
   static __device__ __forceinline__ raw encs(T v, int k, uint32_t key, const Consts<T>& K) {
-    const uint32_t b = __float_as_uint(v * K.scale[k]);
-    const uint32_t t = (b + (round_bits(b, k, key) & 0x1fffu)) & ~0x1fffu;
-    return __float2half_rn(__uint_as_float(t));
+    const float a = v * K.scale[k];
+    half2_sr z = {};
+    z = __builtin_amdgcn_cvt_sr_f16_f32(z, a, round_bits(__float_as_uint(a), k, key), false);
+    _Float16 h = z[0];
+    raw r;
+    __builtin_memcpy(&r, &h, 2);
+    return r;
+  }
+  // Two cells' values of direction k, packed (first cell in the low half).
+  static __device__ __forceinline__ uint32_t encs2(T v0, T v1, int k, uint32_t key0,
+                                                   uint32_t key1, const Consts<T>& K) {
+    const float a0 = v0 * K.scale[k], a1 = v1 * K.scale[k];
+    return sr_half2(a0, round_bits(__float_as_uint(a0), k, key0), a1,
+                    round_bits(__float_as_uint(a1), k, key1));
   }
   static __device__ __forceinline__ double f64(raw r, int k, const d2q9_lattice& L) {
     return L.w[k] + (double)__half2float(r) / L.dev_scale[k];
@@ -208,9 +243,17 @@ template <> struct Codec<D2Q9_STORE_BF16DEV> {
   }
   // Stochastic rounding to bfloat16 (the upper 16 bits of a float).
   static __device__ __forceinline__ raw encs(T v, int k, uint32_t key, const Consts<T>&) {
-    const uint32_t b = __float_as_uint(v);
-    const uint32_t t = b + (round_bits(b, k, key) & 0xffffu);
-    return __ushort_as_bfloat16((unsigned short)(t >> 16));
+    bf162_sr z = {};
+    z = __builtin_amdgcn_cvt_sr_bf16_f32(z, v, round_bits(__float_as_uint(v), k, key), false);
+    __bf16 h = z[0];
+    raw r;
+    __builtin_memcpy(&r, &h, 2);
+    return r;
+  }
+  static __device__ __forceinline__ uint32_t encs2(T v0, T v1, int k, uint32_t key0,
+                                                   uint32_t key1, const Consts<T>&) {
+    return sr_bf162(v0, round_bits(__float_as_uint(v0), k, key0), v1,
+                    round_bits(__float_as_uint(v1), k, key1));
   }
   static __device__ __forceinline__ double f64(raw r, int k, const d2q9_lattice& L) {
     return L.w[k] + (double)__bfloat162float(r);

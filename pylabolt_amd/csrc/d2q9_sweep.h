@@ -170,8 +170,9 @@ __device__ __forceinline__ void gather_block(
 }
 
 // Moments + collision (or nothing for PHASE_STREAM) of one cell, encoded
-// into out[.][q]; rho/u stored when asked.
-template <int S, int MODEL, int FORCING, int PHASE>
+// into out[.][q]; rho/u stored when asked.  With ENC false the collided
+// values stay in g and the caller encodes them (pairs of 16-bit cells).
+template <int S, int MODEL, int FORCING, int PHASE, bool ENC = true>
 __device__ __forceinline__ void finish_cell(
     const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K,
     typename Codec<S>::T (&g)[9], typename Codec<S>::raw (&out)[9][Codec<S>::CPL],
@@ -191,10 +192,12 @@ __device__ __forceinline__ void finish_cell(
     T drho, rho, ux, uy, fx, fy;
     moments<T, FORCING, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
     collide<T, MODEL, FORCING, C::dev>(g, K, drho, rho, ux, uy, fx, fy);
-    // 16-bit storage rounds stochastically (deterministic in the data)
-    const uint32_t key = C::dev && sizeof(typename C::raw) == 2 ? cell_key((uint32_t)c) : 0u;
+    if constexpr (ENC) {
+      // 16-bit storage rounds stochastically (deterministic in the data)
+      const uint32_t key = C::dev && sizeof(typename C::raw) == 2 ? cell_key((uint32_t)c) : 0u;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) out[k][q] = C::encs(g[k], k, key, K);
+      for (int k = 0; k < 9; ++k) out[k][q] = C::encs(g[k], k, key, K);
+    }
     if (store_mom) {
       mdst[c] = rho;
       mdst[L.plane + c] = ux;
@@ -367,10 +370,41 @@ __device__ __forceinline__ void bulk_finish(
     if (L.wall_masks) wall_rows<S, PHASE>(L, K, fsrc, g, p, lane, bp.i, bp.j0);
   }
   typename C::raw out[9][CPL];
+  constexpr bool PAIR = C::dev && sizeof(typename C::raw) == 2 && CPL == 2 &&
+                        PHASE != PHASE_STREAM;
+  if constexpr (PAIR) {
+    // two cells per lane: collide both, then round and pack them together
+    const int64_t c0 = cell_index(L, bp.i, bp.j0);
 #pragma unroll
-  for (int q = 0; q < CPL; ++q)
-    finish_cell<S, MODEL, FORCING, PHASE>(L, K, g[q], out, q, cell_index(L, bp.i, bp.j0 + q),
-                                          write_mom != 0 && !((bp.skip >> q) & 1), mdst);
+    for (int q = 0; q < CPL; ++q)
+      finish_cell<S, MODEL, FORCING, PHASE, false>(L, K, g[q], out, q, c0 + q,
+                                                   write_mom != 0 && !((bp.skip >> q) & 1),
+                                                   mdst);
+    const uint32_t key0 = cell_key((uint32_t)c0), key1 = cell_key((uint32_t)(c0 + 1));
+    uint32_t w[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) w[k] = C::encs2(g[0][k], g[1][k], k, key0, key1, K);
+    if (bp.skip == 0) {
+      const int64_t obase = (((int64_t)bp.i * L.nbj + bp.tj + 1) * 9) * B + lane * CPL;
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        __builtin_nontemporal_store(w[k],
+                                    reinterpret_cast<uint32_t*>(fdst + obase + (int64_t)k * B));
+      return;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const unsigned short h0 = (unsigned short)(w[k] & 0xffffu);
+      const unsigned short h1 = (unsigned short)(w[k] >> 16);
+      __builtin_memcpy(&out[k][0], &h0, 2);
+      __builtin_memcpy(&out[k][1], &h1, 2);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < CPL; ++q)
+      finish_cell<S, MODEL, FORCING, PHASE>(L, K, g[q], out, q, cell_index(L, bp.i, bp.j0 + q),
+                                            write_mom != 0 && !((bp.skip >> q) & 1), mdst);
+  }
   if (bp.skip == 0) {
     store_block<S>(L, fdst, bp.i, bp.tj, lane, out);
   } else {

@@ -93,7 +93,7 @@ class CaseTables:
     """
 
     def __init__(self, nx, ny, solid_pad, elements, x_periodic, y_periodic,
-                 tile_j=256, tile_i=1):
+                 tile_j=256, tile_i=1, solid_tiles="sweep"):
         self.nx, self.ny = nx, ny
         self.x_periodic, self.y_periodic = bool(x_periodic), bool(y_periodic)
         nxp, nyp = nx + 2, ny + 2
@@ -172,9 +172,9 @@ class CaseTables:
                 self.wall_side_pad[i, j] = zg_sides[key]
             elif kind == nt.RULE_BB and key in bb_sides:
                 self.wall_side_pad[i, j] = bb_sides[key]
-        self.set_tile(tile_j, tile_i)
+        self.set_tile(tile_j, tile_i, solid_tiles)
 
-    def set_tile(self, tile_j, tile_i):
+    def set_tile(self, tile_j, tile_i, solid_tiles="sweep"):
         """Classify tiles of tile_j x tile_i interior cells.
 
         Bulk tiles (class 0) hold no solid cell: the bulk sweep treats all
@@ -185,6 +185,12 @@ class CaseTables:
         a partial tile, which are never stored (the ghost column keeps its
         initial zero; rows past the last row are not swept).  General tiles
         (class 1) hold a solid cell.
+
+        ``solid_tiles``: "sweep" lists the general tiles for the general
+        sweep; "cells" lists none and puts their fluid cells in the fix-up
+        list instead (solid cells keep the encoded zero written at
+        initialisation), so a step needs no general sweep; "auto" takes
+        "cells" when at most 1/32 of the tiles are general.
         """
         if tile_j % 64 or not 1 <= tile_i <= 4:
             raise ValueError("tile_j must be 64 * cpl and tile_i 1..4")
@@ -238,9 +244,22 @@ class CaseTables:
         ids = np.arange(self.tile_class.size, dtype=np.int32)
         bulk_ids = ids[self.tile_class == 0]
         general_ids = ids[self.tile_class != 0]
-        self.tile_list = np.concatenate([bulk_ids, general_ids]).astype(np.int32)
-        self.n_bulk, self.n_general = int(bulk_ids.size), int(general_ids.size)
-        # fix-up cells: special cells lying in bulk tiles
+        if solid_tiles not in ("sweep", "cells", "auto"):
+            raise ValueError("solid_tiles must be 'sweep', 'cells' or 'auto'")
+        self.solid_tiles_req = solid_tiles
+        mode = solid_tiles
+        if mode == "auto":
+            mode = "cells" if 32 * general_ids.size <= ids.size else "sweep"
+        self.solid_tiles = mode
+        if mode == "sweep":
+            self.tile_list = np.concatenate([bulk_ids, general_ids]).astype(np.int32)
+            self.n_general = int(general_ids.size)
+        else:
+            self.tile_list = bulk_ids.astype(np.int32)
+            self.n_general = 0
+        self.n_bulk = int(bulk_ids.size)
+        # fix-up cells: special cells lying in bulk tiles ("cells": and every
+        # fluid cell of a tile holding a solid)
         in_bulk = np.repeat(np.repeat(tiles, self.tile_i, axis=0),
                             self.tile_j, axis=1)[:nx, :ny]
         # masked bits per (bulk tile, row, 64-cell word): special cells
@@ -272,7 +291,10 @@ class CaseTables:
         self.wall_index = np.where(has, np.cumsum(has) - 1, -1).astype(np.int32)
         self.wall_masks = (np.ascontiguousarray(zmask[has]).reshape(-1)
                            if self.n_wall else None)
-        ii, jj = np.nonzero(special & in_bulk)
+        fixed = special & in_bulk
+        if mode == "cells":
+            fixed |= fluid & ~in_bulk
+        ii, jj = np.nonzero(fixed)
         pitch = int(math.ceil((ny + 2) / ALIGN) * ALIGN)
         self.fix_cells = (ORIGIN + (ii + 1) * pitch + (jj + 1)).astype(np.int32)
         if self.fix_cells.size == 0:
@@ -345,7 +367,7 @@ class D2Q9Engine:
                  x_periodic, y_periodic, model, forcing, omega, gravity,
                  inv_cs_2, inv_cs_4, weights, float_min, storage="f64",
                  force_field=None, tile=(64, 4), rates=None, dev_scale=None,
-                 device=None, graph_steps=16, stream=None):
+                 device=None, graph_steps=16, stream=None, solid_tiles=None):
         self.lib = nt.lib()
         self.device = (torch.device(device) if device is not None else
                        torch.device("cuda", torch.cuda.current_device()))
@@ -361,8 +383,15 @@ class D2Q9Engine:
         self.B, self.nbj, self.pop_len = pop_geometry(self.nx, self.ny,
                                                       self.cpl)
         self.tile_i = int(tile[1]) if tile is not None else 4
+        if solid_tiles is None:
+            # 16-bit steps run as one kernel when the tiles holding solids
+            # are few (their fluid cells go through the fix-up path)
+            solid_tiles = ("auto" if self.storage in (nt.STORE_F16DEV,
+                                                      nt.STORE_BF16DEV)
+                           else "sweep")
         self.tables = CaseTables(self.nx, self.ny, solid_pad, elements,
-                                 x_periodic, y_periodic, self.B, self.tile_i)
+                                 x_periodic, y_periodic, self.B, self.tile_i,
+                                 solid_tiles)
         self.stream = (stream if stream is not None else
                        torch.cuda.Stream(device=self.device))
         self._stream_ptr = ctypes.c_void_p(self.stream.cuda_stream)
@@ -482,7 +511,7 @@ class D2Q9Engine:
             return
         self.stream.synchronize()
         self.close()
-        self.tables.set_tile(self.B, tile_i)
+        self.tables.set_tile(self.B, tile_i, self.tables.solid_tiles_req)
         self.tile_i = tile_i
         self._upload_tiles()
 

@@ -62,13 +62,15 @@ class Solver:
     ``simulation`` defaults to ``simulation.py`` in the working directory,
     as in the reference.  Extra keywords select the population storage
     (``"f64"`` is the reference-exact path), the launch tile, the HIP
-    graph chunk and the torch stream the engine runs on (a new one if
-    None).
+    graph chunk, the torch stream the engine runs on (a new one if None)
+    and how tiles holding solid cells are swept (``solid_tiles``: "sweep",
+    "cells" or "auto"; None picks per storage form).
     """
 
     def __init__(self, comm=None, backend="gpu", n_threads=None,
                  simulation=None, storage="f64", tile=(64, 4),
-                 graph_steps=16, verbose=True, output_root=".", stream=None):
+                 graph_steps=16, verbose=True, output_root=".", stream=None,
+                 solid_tiles=None):
         comm = comm if comm is not None else SingleRankComm()
         mpi_rank = comm.Get_rank()
         self.verbose = verbose
@@ -80,7 +82,8 @@ class Solver:
         self.state = State(simulation, comm, mpi_rank, fluid=True)
         self.backend = Backend(comm, self.state, backend, n_threads,
                                storage=storage, tile=tile,
-                               graph_steps=graph_steps, stream=stream)
+                               graph_steps=graph_steps, stream=stream,
+                               solid_tiles=solid_tiles)
         self.mpi_operator = MPIOperator(comm, self.state)
         self.collision_operator = CollisionOperator(simulation, self.model,
                                                     self.state, comm,

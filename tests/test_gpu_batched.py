@@ -70,7 +70,10 @@ def test_batched_fp32_bitwise_equals_per_lattice(tile_i, policy, grid, chunk,
     print(f"[batched] f32 {cfg} lattices "
           f"{[tuple(s.mesh_dict['grid']) for s in sims]}", flush=True)
     batch = BatchedLattices(sims, storage="f32", config=cfg, graph_steps=8)
-    assert batch.has_general == int(obstacle)
+    assert batch.has_general == int(any(e.tables.n_general > 0
+                                        for e in batch.engines))
+    assert any((e.tables.tile_class != 0).any()
+               for e in batch.engines) == obstacle
     solo = [_solo(s, "f32") for s in _mixed_sims(obstacle=obstacle)]
     done = 0
     for n in (1, 1, 19, 17):

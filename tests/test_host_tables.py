@@ -138,3 +138,40 @@ def test_wall_rows_swept_inline(name, B):
         # the lid (moving wall) and the corners stay in the list
         lid = set(E.ORIGIN + (i + 1) * pitch + ny for i in range(nx))
         assert lid <= fix
+
+
+@pytest.mark.parametrize("B", [64, 128])
+def test_solid_tiles_as_fix_cells(B):
+    """"cells" mode lists no general tile: the fix-up list gains every fluid
+    cell of the tiles holding a solid, the bulk tiles and their masks are
+    unchanged; "auto" takes "cells" while at most 1/32 of the tiles are
+    general."""
+    st = State(CASES["cylinder"]())
+    nx, ny = (int(v) for v in st.mesh.grid_global_shape)
+    args = (nx, ny, st.fields.solid, E.elements_from_boundary(st.boundary),
+            st.boundary.x_periodic, st.boundary.y_periodic, B, 4)
+    sweep = E.CaseTables(*args, solid_tiles="sweep")
+    cells = E.CaseTables(*args, solid_tiles="cells")
+    assert sweep.n_general > 0 and cells.n_general == 0
+    assert cells.n_bulk == sweep.n_bulk
+    assert np.array_equal(cells.tile_list, sweep.tile_list[:sweep.n_bulk])
+    assert np.array_equal(cells.bulk_masks, sweep.bulk_masks)
+    pitch, _ = E.geometry(nx, ny)
+    general = np.zeros((nx, ny), dtype=bool)
+    for tid in sweep.tile_list[sweep.n_bulk:]:
+        tj, ti = int(tid) % sweep.ntj, int(tid) // sweep.ntj
+        general[ti * 4:ti * 4 + 4, tj * B:tj * B + B] = True
+    fluid = (sweep.flags_pad[1:-1, 1:-1] & 1) != 0
+    ii, jj = np.nonzero(general & fluid)
+    extra = set((E.ORIGIN + (ii + 1) * pitch + (jj + 1)).tolist())
+    before = set(sweep.fix_cells[:sweep.n_fix].tolist())
+    after = cells.fix_cells[:cells.n_fix]
+    assert np.all(np.diff(after) > 0)          # sorted, no duplicates
+    assert set(after.tolist()) == before | extra
+    assert not (before & extra)
+    auto = E.CaseTables(*args, solid_tiles="auto")
+    n_tiles = sweep.n_bulk + sweep.n_general
+    want = "cells" if 32 * sweep.n_general <= n_tiles else "sweep"
+    assert auto.solid_tiles == want
+    with pytest.raises(ValueError):
+        E.CaseTables(*args, solid_tiles="tiles")
