@@ -242,13 +242,12 @@ template <int S, int PHASE>
 __device__ __forceinline__ void wall_rows(const d2q9_lattice& L,
                                           const Consts<typename Codec<S>::T>& K,
                                           const typename Codec<S>::raw* __restrict__ fsrc,
-                                          typename Codec<S>::T (&g)[Codec<S>::CPL][9], int p,
+                                          typename Codec<S>::T (&g)[Codec<S>::CPL][9], int slot,
                                           int lane, int i, int j0) {
   using T = typename Codec<S>::T;
   constexpr int CPL = Codec<S>::CPL;
-  const int slot = L.wall_index[p];
-  if (slot < 0) return;
-  const uint64_t* zrow = L.wall_masks + ((int64_t)slot * blockDim.y + threadIdx.y) * 6 * CPL;
+  const int row = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const uint64_t* zrow = L.wall_masks + ((int64_t)slot * blockDim.y + row) * 6 * CPL;
   bool any[6];
 #pragma unroll
   for (int s = 0; s < 6; ++s) {
@@ -320,35 +319,49 @@ __device__ __forceinline__ void wall_rows(const d2q9_lattice& L,
 // the lane, and the lane's masked cells.
 struct BulkPos {
   int i, tj, j0;
-  uint32_t skip;
+  uint32_t skip;  // masked cells of the lane (bit q: cell j0 + q)
+  int wslot;      // the tile's slot in the wall masks, or -1
 };
 
+// Row, block and first cell of the lane for bulk tile p.  Each wave sweeps
+// one row of the tile (workgroup 64 x rows).
 template <int S>
-__device__ __forceinline__ BulkPos bulk_pos(const d2q9_lattice& L, int ntj, int p) {
+__device__ __forceinline__ BulkPos bulk_place(const d2q9_lattice& L, int ntj, int p) {
   constexpr int CPL = Codec<S>::CPL;
   constexpr int B = 64 * CPL;
   BulkPos bp;
   const int t = L.tiles[p];
   bp.tj = t % ntj;
   const int ti = t / ntj;
-  const int lane = (int)threadIdx.x;
   bp.i = 1 + ti * (int)blockDim.y + (int)threadIdx.y;
-  bp.j0 = 1 + bp.tj * B + lane * CPL;
+  bp.j0 = 1 + bp.tj * B + (int)threadIdx.x * CPL;
   bp.skip = 0;
-  if (bp.i > L.nx) return bp;  // whole wave (last tile row of a partial tile)
-  // masked bits of this lane's cells
-  const uint64_t* mrow = L.bulk_masks + ((int64_t)p * blockDim.y + threadIdx.y) * CPL;
-  bool any = false;
-#pragma unroll
-  for (int w = 0; w < CPL; ++w) any |= mrow[w] != 0;
-  if (any) {
-#pragma unroll
-    for (int q = 0; q < CPL; ++q) {
-      const int o = lane * CPL + q;
-      bp.skip |= (uint32_t)((mrow[o >> 6] >> (o & 63)) & 1ull) << q;
-    }
-  }
+  bp.wslot = -1;
   return bp;
+}
+
+// The wave's mask words (uniform over the wave: scalar loads), issued
+// before the gather so that their latency overlaps it.
+template <int S>
+__device__ __forceinline__ void bulk_mask_load(const d2q9_lattice& L, int p,
+                                               uint64_t (&m)[Codec<S>::CPL]) {
+  constexpr int CPL = Codec<S>::CPL;
+  const int row = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const uint64_t* mrow = L.bulk_masks + ((int64_t)p * blockDim.y + row) * CPL;
+#pragma unroll
+  for (int w = 0; w < CPL; ++w) m[w] = mrow[w];
+}
+
+template <int S>
+__device__ __forceinline__ uint32_t bulk_skip(const uint64_t (&m)[Codec<S>::CPL], int lane) {
+  constexpr int CPL = Codec<S>::CPL;
+  uint32_t skip = 0;
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const int o = lane * CPL + q;
+    skip |= (uint32_t)((m[o >> 6] >> (o & 63)) & 1ull) << q;
+  }
+  return skip;
 }
 
 // Everything after the loads of one bulk tile: decode and shuffle, the
@@ -367,7 +380,7 @@ __device__ __forceinline__ void bulk_finish(
   T g[CPL][9];
   gather_finish<S, PHASE>(K, rb, lane, g);
   if constexpr (PHASE != PHASE_COLLIDE) {
-    if (L.wall_masks) wall_rows<S, PHASE>(L, K, fsrc, g, p, lane, bp.i, bp.j0);
+    if (bp.wslot >= 0) wall_rows<S, PHASE>(L, K, fsrc, g, bp.wslot, lane, bp.i, bp.j0);
   }
   typename C::raw out[9][CPL];
   constexpr bool PAIR = C::dev && sizeof(typename C::raw) == 2 && CPL == 2 &&
@@ -429,11 +442,16 @@ __device__ __forceinline__ void bulk_tile(
     typename Codec<S>::raw* __restrict__ fdst,
     typename Codec<S>::T* __restrict__ mdst, int write_mom, int ntj, int p) {
   using T = typename Codec<S>::T;
-  const BulkPos bp = bulk_pos<S>(L, ntj, p);
-  if (bp.i > L.nx) return;
+  BulkPos bp = bulk_place<S>(L, ntj, p);
+  if (bp.i > L.nx) return;  // whole wave (last tile row of a partial tile)
+  // tile tables first, then the gather: none of the loads waits on another
+  uint64_t m[Codec<S>::CPL];
+  bulk_mask_load<S>(L, p, m);
+  if (PHASE != PHASE_COLLIDE && L.wall_masks) bp.wslot = L.wall_index[p];
   const Consts<T> K = load_consts<T>(L);
   RawBlock<S> rb;
   gather_issue<S, PHASE>(L, fsrc, bp.i, bp.tj, (int)threadIdx.x, bp.j0, rb);
+  bp.skip = bulk_skip<S>(m, (int)threadIdx.x);
   bulk_finish<S, MODEL, FORCING, PHASE>(L, K, rb, bp, fsrc, fdst, mdst, write_mom, p);
 }
 
