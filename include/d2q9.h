@@ -135,14 +135,17 @@ typedef struct d2q9_lattice {
   double dev_scale[9];          /* 16-bit storage scale per direction */
   d2q9_fconst cf;
   const uint8_t* flags;         /* [plane] */
-  /* Tile lists: tiles[0 .. n_bulk) are bulk tiles (every cell fluid and
-   * inside the lattice), tiles[n_bulk .. n_bulk + n_general) the others.
-   * Tile t covers rows 1 + (t / ntj) * tile_i .. and cells
-   * 1 + (t % ntj) * tile_j .., ntj = ceil(ny / tile_j).  NULL: every tile
-   * is handled as general.  The bulk sweep treats every cell as plain
-   * interior fluid; fix_cells lists (as origin + i * pitch + j) the cells
-   * of bulk tiles that are not -- solid neighbour, boundary rule, stored
-   * moment, periodic edge -- and a second pass recomputes them. */
+  /* Tile lists: tiles[0 .. n_bulk) are bulk tiles (no solid cell; cells
+   * past the lattice are masked), tiles[n_bulk .. n_bulk + n_general)
+   * general tiles (holding a solid cell).  Tile t covers rows
+   * 1 + (t / ntj) * tile_i .. and cells 1 + (t % ntj) * tile_j ..,
+   * ntj = ceil(ny / tile_j).  NULL: every tile is handled as general.  The
+   * bulk sweep treats every cell as plain interior fluid; fix_cells lists
+   * (as origin + i * pitch + j) the cells of bulk tiles that are not --
+   * solid neighbour, boundary rule, stored moment, periodic edge -- and a
+   * second pass recomputes them.  A tile in neither list (n_bulk +
+   * n_general may be less than the tile count) has its fluid cells in
+   * fix_cells; its solid cells keep the encoded zero d2q9_init_eq wrote. */
   const int32_t* tiles;
   int32_t n_bulk, n_general;
   const int32_t* fix_cells;
@@ -190,13 +193,15 @@ int d2q9_collide(const d2q9_lattice* L, const void* f_src, void* f_dst,
 int d2q9_stream_bc(const d2q9_lattice* L, const void* f_src, void* f_dst,
                    const void* mom_src, void* stream);
 
-/* The sweeps below launch the bulk tiles on `stream` and the general tiles
- * plus fix-up cells on an auxiliary stream of the same device, forked from
- * and joined back to `stream` with events (both passes write disjoint
- * cells), so the two overlap; the library creates that stream and its two
- * events on first use per device and keeps them for the process.  Work is
- * ordered on `stream` exactly as if it ran there alone, including under
- * HIP graph capture. */
+/* The sweeps (d2q9_collide, d2q9_stream_bc, d2q9_fused_step) launch the
+ * bulk tiles on `stream` and the general tiles plus fix-up cells on an
+ * auxiliary stream of the same device, forked from and joined back to
+ * `stream` with events (both passes write disjoint cells), so the two
+ * overlap; the library creates that stream and its two events on first use
+ * per device and keeps them for the process.  Work is ordered on `stream`
+ * exactly as if it ran there alone, including under HIP graph capture.
+ * Exception: a fused step in FP32 storage, or in FP16/BF16 storage with
+ * n_general == 0, is one kernel on `stream`. */
 
 /* Fused: stream + boundaries of step n, moments + collision of step n+1. */
 int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,

@@ -68,6 +68,7 @@ __device__ constexpr int ROWNORM[9] = {9, 36, 36, 6, 12, 6, 12, 4, 4};
 template <typename T> struct Consts {
   T w[9], rates[9], scale[9], iscale[9];
   T om, ics2, ics4, fmin, gx, gy;
+  T nzero;  // -0, computed at run time (see Codec<D2Q9_STORE_F16DEV>::dec)
 };
 
 template <typename T>
@@ -89,6 +90,7 @@ __device__ __forceinline__ Consts<double> load_consts<double>(const d2q9_lattice
   K.fmin = L.float_min;
   K.gx = L.gravity[0];
   K.gy = L.gravity[1];
+  K.nzero = -0.0;
   return K;
 }
 
@@ -108,6 +110,9 @@ __device__ __forceinline__ Consts<float> load_consts<float>(const d2q9_lattice& 
   K.fmin = L.cf.float_min;
   K.gx = L.cf.gravity[0];
   K.gy = L.cf.gravity[1];
+  // -0 * (a positive scale) = -0; formed at run time, so the compiler keeps
+  // the additions of it (x + -0 = x exactly)
+  K.nzero = -0.0f * K.iscale[0];
   return K;
 }
 
@@ -115,7 +120,8 @@ __device__ __forceinline__ Consts<float> load_consts<float>(const d2q9_lattice& 
 // Storage codecs: raw HBM element <-> compute value.
 //   dev == false: the value is f_k (FP64 reference path)
 //   dev == true:  the value is d_k = f_k - w_k; raw = d_k * scale_k
-// CPL: cells per lane, chosen so that one lane moves 8 bytes per direction.
+// CPL: cells per lane (FP64 1: 8 bytes per lane and direction; FP32 2:
+// 8 bytes; 16-bit 2: 4 bytes).
 // ---------------------------------------------------------------------------
 // 32-bit integer hash ("lowbias32" finaliser, C. Wellons, 2018): xor-shift
 // and multiply rounds with full avalanche.
@@ -202,8 +208,11 @@ template <> struct Codec<D2Q9_STORE_F16DEV> {
   using T = float;
   static constexpr bool dev = true;
   static constexpr int CPL = CPL16;
+  // h * iscale + (-0) is h * iscale exactly; written as a fused multiply-
+  // add with a run-time addend it becomes one v_fma_mix_f32, which reads
+  // the half operand (either half of a register) directly.
   static __device__ __forceinline__ T dec(raw r, int k, const Consts<T>& K) {
-    return __half2float(r) * K.iscale[k];
+    return __builtin_fmaf(__half2float(r), K.iscale[k], K.nzero);
   }
   static __device__ __forceinline__ raw enc(T v, int k, const Consts<T>& K) {
     return __float2half_rn(v * K.scale[k]);
