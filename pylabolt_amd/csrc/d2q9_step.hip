@@ -1,18 +1,21 @@
 // Single-lattice D2Q9 sweep kernels and their C ABI (include/d2q9.h).
 //
 // A wave owns one AoSoA block (B = 64 * CPL cells along y) of one row; a
-// workgroup stacks tile_i such rows.  Each lane moves 8 bytes per direction
-// (1 double, 2 floats or 4 halves), reads the nine directions at their pull
-// offsets (the y-shift crosses lanes by a shuffle when CPL > 1) and writes
-// them back unshifted with non-temporal stores, so every HBM access of a
-// wave is one contiguous segment and the nine directions of a block sit
-// side by side.
+// workgroup stacks tile_i such rows.  Each lane moves one word per
+// direction (1 double, 2 floats or 2 halves), reads the nine directions at
+// their pull offsets (the y-shift crosses lanes by a shuffle when CPL > 1)
+// and writes them back unshifted with non-temporal stores, so every HBM
+// access of a wave is one contiguous segment and the nine directions of a
+// block sit side by side.
 //
-// Tiles whose cells are all plain fluid (no solid neighbour, no boundary
-// rule, no periodic edge, no moment the next step reads, fully inside the
-// lattice) take a branch-free path that never touches the flag array; the
-// others handle each cell with its flag byte: bounce-back, periodic wrap and
-// boundary rules in the same sweep.
+// Bulk tiles (no solid cell) sweep every cell as plain interior fluid and
+// never read the flag array; straight-wall rows are done in the tile from
+// per-tile masks, and the other special cells (solid neighbour, boundary
+// rule, periodic edge, a moment the next step reads) are masked out of the
+// store and recomputed one per thread by the fix-up pass, with their flag
+// byte.  General tiles (holding a solid) handle each cell with its flag
+// byte.  FP64 runs the bulk tiles and the general/fix-up pass as two
+// overlapped kernels; the float forms run a fused step as one kernel.
 #include <hip/hip_runtime.h>
 
 #include <cstddef>

@@ -2,18 +2,22 @@
 
 HBM layout (one lattice):
 
-* populations: two buffers (ping-pong), each ``9 * plane`` elements of the
-  storage type, structure of arrays: direction ``k`` occupies
-  ``[k * plane, (k + 1) * plane)``;
-* inside a plane, padded cell ``(i, j)`` (``0 <= i <= nx+1``,
-  ``0 <= j <= ny+1``, ghost ring included) sits at
-  ``origin + i * pitch + j`` with ``origin = 63`` and ``pitch`` a multiple
-  of 64, so each interior row ``j = 1..ny`` starts on a 64-element (512 B
-  in FP64) boundary;
+* populations: two buffers (ping-pong) of the storage type, blocked
+  (array of structures of arrays): padded cell ``(i, j)`` (``0 <= i <=
+  nx+1``, ``0 <= j <= ny+1``, ghost ring included), direction ``k`` sits
+  at ``((i * nbj + jj // B) * 9 + k) * B + jj % B`` with ``jj = j + B - 1``
+  and ``B = 64 * cpl`` cells per block (``cpl`` 1 for FP64, 2 otherwise),
+  so interior cell ``j = 1`` starts a block and one wave reads or writes
+  one contiguous block of one direction;
+* side planes (flags, boundary slots, moments, force field): padded cell
+  ``(i, j)`` at ``origin + i * pitch + j`` with ``origin = 63`` and
+  ``pitch`` a multiple of 64;
 * moments: two buffers of three planes ``[rho | ux | uy]`` in the compute
   type (double for FP64 storage, float otherwise);
-* one flag byte per padded cell, a per-tile class byte, and the boundary
-  tables (a slot per boundary node, one rule per direction).
+* one flag byte per padded cell, the boundary tables (a slot per boundary
+  node, one rule per direction), and the tile tables of ``CaseTables``:
+  tile list, masked-cell bits of the bulk tiles, straight-wall masks and
+  the fix-up cell list.
 
 Step bookkeeping.  The reference step n computes moments and collision from
 ``pop_new[n-1]`` into ``pop[n]``, then streams and applies boundaries into
