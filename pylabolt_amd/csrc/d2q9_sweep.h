@@ -66,6 +66,28 @@ __device__ __forceinline__ float wave_readlane<float>(float v, int lane) {
   return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v), lane));
 }
 
+// Whole-wave shifts by one lane (GFX9 DPP wave_shr:1 / wave_shl:1): lane l
+// receives the value of lane l - 1 (from_prev) or l + 1 (from_next); the
+// lane without a source (0, resp. 63) keeps its own value, as __shfl_up /
+// __shfl_down do.  One VALU move per dword instead of an LDS-crossbar
+// permute and its wait.
+template <int CTRL, typename T>
+__device__ __forceinline__ T wave_shift(T v) {
+  static_assert(sizeof(T) % 4 == 0, "dword-sized values");
+  constexpr int N = sizeof(T) / 4;
+  int w[N];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int n = 0; n < N; ++n) w[n] = __builtin_amdgcn_update_dpp(w[n], w[n], CTRL, 0xf, 0xf, false);
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T wave_from_prev(T v) { return wave_shift<0x138>(v); }
+template <typename T>
+__device__ __forceinline__ T wave_from_next(T v) { return wave_shift<0x130>(v); }
+
 // Raw words of one lane's gather (one per direction) and, for CPL > 1,
 // the halo element; loaded by gather_issue, decoded by gather_finish.
 template <int S>
@@ -144,12 +166,12 @@ __device__ __forceinline__ void gather_finish(const Consts<typename Codec<S>::T>
         const int h = k == 2 ? 0 : k == 5 ? 1 : k == 6 ? 2 : k == 4 ? 3 : k == 7 ? 4 : 5;
         const T hv = wave_readlane<T>(C::dec(rb.halo, k, K), h);
         if (cy == 1) {  // cell j takes j - 1
-          T prev = __shfl_up(x[CPL - 1], 1, 64);
+          T prev = wave_from_prev(x[CPL - 1]);
           g[0][k] = lane == 0 ? hv : prev;
 #pragma unroll
           for (int q = 1; q < CPL; ++q) g[q][k] = x[q - 1];
         } else {  // cell j takes j + 1
-          T next = __shfl_down(x[0], 1, 64);
+          T next = wave_from_next(x[0]);
 #pragma unroll
           for (int q = 0; q < CPL - 1; ++q) g[q][k] = x[q + 1];
           g[CPL - 1][k] = lane == 63 ? hv : next;
@@ -271,7 +293,7 @@ __device__ __forceinline__ void wall_rows(const d2q9_lattice& L,
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
       const int d = DU[n];
-      const T nxt = __shfl_down(g[0][d], 1, 64);
+      const T nxt = wave_from_next(g[0][d]);
 #pragma unroll
       for (int q = 0; q < CPL; ++q) {
         const T src = (q + 1 < CPL) ? g[q + 1][d] : nxt;
@@ -284,7 +306,7 @@ __device__ __forceinline__ void wall_rows(const d2q9_lattice& L,
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
       const int d = DD[n];
-      const T prv = __shfl_up(g[CPL - 1][d], 1, 64);
+      const T prv = wave_from_prev(g[CPL - 1][d]);
 #pragma unroll
       for (int q = CPL - 1; q >= 0; --q) {
         const T src = (q > 0) ? g[q - 1][d] : prv;
