@@ -208,6 +208,31 @@ int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
                     const void* mom_src, void* mom_dst, int write_moments,
                     void* stream);
 
+/* Two fused steps (FP64 storage): f_dst and mom_dst as after
+ * d2q9_fused_step(f_src -> f_tmp) and d2q9_fused_step(f_tmp -> f_dst),
+ * bitwise.  It has no counterpart in the reference: it replaces two of its
+ * time steps (solvers/fluidLB.py:206-253) by one sweep over the interior.
+ * deep_tiles lists tiles of d2q9_deep_rows() rows x 64 cells (tile t: rows
+ * 1 + (t / ntj) * rows .., cells 1 + (t % ntj) * 64 .., ntj =
+ * ceil(ny / 64)) whose cells and whose neighbours one cell deep are plain
+ * fluid (flag D2Q9_CELL_FLUID only, no periodic edge); the caller
+ * guarantees it.  Each is swept once for both steps, step n + 1 of the tile
+ * grown by one cell held in LDS.  L_rim_a and L_rim_b describe the same
+ * lattice as L with tile and fix-up lists restricted to the rest (the rim):
+ * L_rim_b covers every cell outside the deep tiles, L_rim_a those cells and
+ * also every cell they pull from.  The rim runs as two single steps through
+ * f_tmp / mom_tmp (scratch of the sizes of f_src / mom_src) on the
+ * auxiliary stream, forked and joined as for d2q9_fused_step.  All three
+ * population buffers and all three moment buffers must differ. */
+int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
+                     const d2q9_lattice* L_rim_b, const int32_t* deep_tiles,
+                     int32_t n_deep, const void* f_src, void* f_dst, void* f_tmp,
+                     const void* mom_src, void* mom_dst, void* mom_tmp,
+                     int write_moments, void* stream);
+
+/* Rows of a deep tile of d2q9_fused_step2. */
+int d2q9_deep_rows(void);
+
 /* rho/u of the populations f (e.g. pop_new) on every fluid cell. */
 int d2q9_moments(const d2q9_lattice* L, const void* f, void* mom,
                  void* stream);

@@ -14,7 +14,7 @@ from pylabolt_amd.engine import D2Q9Engine, elements_from_boundary
 class Backend:
     def __init__(self, comm, state, backend_type="gpu", n_threads=None,
                  storage="f64", tile=(64, 4), device=None, graph_steps=16,
-                 stream=None, solid_tiles=None):
+                 stream=None, solid_tiles=None, two_step=None):
         if backend_type != "gpu":
             raise ValueError("Unsupported backend: " + str(backend_type) +
                              " (this engine runs on the GPU only)")
@@ -26,6 +26,7 @@ class Backend:
         self.graph_steps = graph_steps
         self.stream = stream
         self.solid_tiles = solid_tiles
+        self.two_step = two_step
         self.lib = nt.lib()
         self.device = torch.device(device) if device else torch.device(
             "cuda", torch.cuda.current_device())
@@ -46,5 +47,6 @@ class Backend:
             force_field=force_field, tile=self.tile,
             rates=collision_operator.rates, dev_scale=dev_scale,
             device=self.device, graph_steps=self.graph_steps,
-            stream=self.stream, solid_tiles=self.solid_tiles)
+            stream=self.stream, solid_tiles=self.solid_tiles,
+            two_step=self.two_step)
         return self.engine

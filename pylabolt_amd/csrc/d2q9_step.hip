@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <initializer_list>
 #include <mutex>
 #include <cstdio>
 #include <cstring>
@@ -99,6 +100,84 @@ __global__ __launch_bounds__(256) void k_step_all(
   } else {
     bulk_tile<S, MODEL, FORCING, PHASE>(L, fsrc, fdst, mdst, write_mom, ntj,
                                         xcd_remap(b - ngen - nfixb, L.n_bulk));
+  }
+}
+
+// Two fused steps of an FP64 deep tile: DEEP_ROWS rows x 64 cells whose
+// neighbourhood two cells deep holds only plain fluid cells (the host
+// checks it).  Stage one computes step n + 1 (pull from the lattice,
+// moments, collision) on the tile grown by one cell on every side into LDS;
+// stage two pulls step n + 1 from LDS and stores step n + 2.  Per cell and
+// two steps the lattice is read about 1.2 times (the grown rows and
+// columns) and written once, against twice each for two single steps.
+// Waves 0 .. DEEP_ROWS + 1 sweep the grown rows (lane l: cell j0 + l); the
+// last wave computes the grown columns j0 - 1 and j0 + 64 of every grown
+// row (lane 2r + side).  Arithmetic per cell is that of the single step.
+constexpr int DEEP_ROWS = 12;
+constexpr int DEEP_R1 = DEEP_ROWS + 2;
+constexpr int DEEP_WAVES = DEEP_R1 + 1;
+
+template <int MODEL, int FORCING>
+__device__ __forceinline__ void deep_collide(const d2q9_lattice& L, const Consts<double>& K,
+                                             double (&g)[9], int64_t c, double& rho,
+                                             double& ux, double& uy) {
+  double ffx = 0.0, ffy = 0.0;
+  if (L.force_source == D2Q9_FSRC_FIELD) {
+    const double* ff = static_cast<const double*>(L.force_field);
+    ffx = ff[c];
+    ffy = ff[L.plane + c];
+  }
+  double drho, fx, fy;
+  moments<double, FORCING, false>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
+  collide<double, MODEL, FORCING, false>(g, K, drho, rho, ux, uy, fx, fy);
+}
+
+template <int MODEL, int FORCING>
+__global__ __launch_bounds__(64 * DEEP_WAVES) void k_deep2(
+    const d2q9_lattice L, const int32_t* __restrict__ deep, int n_deep, int ntj,
+    const double* __restrict__ fsrc, double* __restrict__ fdst, double* __restrict__ mdst,
+    int write_mom) {
+  __shared__ double s1[DEEP_R1][9][66];
+  const int lane = (int)threadIdx.x, w = (int)threadIdx.y;
+  const int t = deep[xcd_remap((int)blockIdx.x, n_deep)];
+  const int i0 = 1 + (t / ntj) * DEEP_ROWS, j0 = 1 + (t % ntj) * 64;
+  const Consts<double> K = load_consts<double>(L);
+  {
+    int r = w, cl = lane + 1;
+    bool act = true;
+    if (w == DEEP_R1) {
+      r = lane >> 1;
+      cl = (lane & 1) ? 65 : 0;
+      act = lane < 2 * DEEP_R1;
+    }
+    if (act) {
+      const int i = i0 - 1 + r, j = j0 - 1 + cl;
+      double g[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) g[k] = fsrc[pop_index<64>(L, i - CX[k], j - CY[k], k)];
+      double rho, ux, uy;
+      deep_collide<MODEL, FORCING>(L, K, g, cell_index(L, i, j), rho, ux, uy);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) s1[r][k][cl] = g[k];
+    }
+  }
+  __syncthreads();
+  if (w >= 1 && w <= DEEP_ROWS) {
+    const int i = i0 - 1 + w, j = j0 + lane;
+    double g[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g[k] = s1[w - CX[k]][k][lane + 1 - CY[k]];
+    const int64_t c = cell_index(L, i, j);
+    double rho, ux, uy;
+    deep_collide<MODEL, FORCING>(L, K, g, c, rho, ux, uy);
+    const int64_t o = pop_index<64>(L, i, j, 0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) store_nt(g[k], fdst + o + (int64_t)k * 64);
+    if (write_mom) {
+      mdst[c] = rho;
+      mdst[L.plane + c] = ux;
+      mdst[2 * L.plane + c] = uy;
+    }
   }
 }
 
@@ -329,9 +408,12 @@ int aux_for_current_device(Aux** out) {
   return 0;
 }
 
+// fork == false: the general/fix-up kernel runs on `s` too, before the
+// bulk kernel (for callers that already run on the auxiliary stream).
 template <int S, int MODEL, int FORCING, int PHASE>
 int launch_sweep(const d2q9_lattice* L, const void* fsrc, void* fdst,
-                 const void* msrc, void* mdst, int write_mom, hipStream_t s) {
+                 const void* msrc, void* mdst, int write_mom, hipStream_t s,
+                 bool fork = true) {
   using C = Codec<S>;
   const int ntj = (L->ny + L->tile_j - 1) / L->tile_j;
   const int nti = (L->nx + L->tile_i - 1) / L->tile_i;
@@ -377,7 +459,7 @@ int launch_sweep(const d2q9_lattice* L, const void* fsrc, void* fdst,
   }
   hipStream_t aux = s;
   Aux* a = nullptr;
-  if (L->n_bulk > 0 && L->n_general + nfixb > 0) {
+  if (fork && L->n_bulk > 0 && L->n_general + nfixb > 0) {
     if (int r = aux_for_current_device(&a)) return r;
     aux = a->stream;
     if (int r = check_hip(hipEventRecord(a->fork, s), "hipEventRecord fork")) return r;
@@ -405,15 +487,15 @@ int launch_sweep(const d2q9_lattice* L, const void* fsrc, void* fdst,
 
 template <int S, int PHASE>
 int dispatch_model(const d2q9_lattice* L, const void* a, void* b, const void* c,
-                   void* d, int wm, hipStream_t s) {
+                   void* d, int wm, hipStream_t s, bool fork = true) {
   const int key = L->collision * 3 + L->forcing;
   switch (key) {
-    case 0: return launch_sweep<S, D2Q9_BGK, D2Q9_FORCING_NONE, PHASE>(L, a, b, c, d, wm, s);
-    case 1: return launch_sweep<S, D2Q9_BGK, D2Q9_GUO_LINEAR, PHASE>(L, a, b, c, d, wm, s);
-    case 2: return launch_sweep<S, D2Q9_BGK, D2Q9_GUO_SECOND_ORDER, PHASE>(L, a, b, c, d, wm, s);
-    case 3: return launch_sweep<S, D2Q9_MRT, D2Q9_FORCING_NONE, PHASE>(L, a, b, c, d, wm, s);
-    case 4: return launch_sweep<S, D2Q9_MRT, D2Q9_GUO_LINEAR, PHASE>(L, a, b, c, d, wm, s);
-    case 5: return launch_sweep<S, D2Q9_MRT, D2Q9_GUO_SECOND_ORDER, PHASE>(L, a, b, c, d, wm, s);
+    case 0: return launch_sweep<S, D2Q9_BGK, D2Q9_FORCING_NONE, PHASE>(L, a, b, c, d, wm, s, fork);
+    case 1: return launch_sweep<S, D2Q9_BGK, D2Q9_GUO_LINEAR, PHASE>(L, a, b, c, d, wm, s, fork);
+    case 2: return launch_sweep<S, D2Q9_BGK, D2Q9_GUO_SECOND_ORDER, PHASE>(L, a, b, c, d, wm, s, fork);
+    case 3: return launch_sweep<S, D2Q9_MRT, D2Q9_FORCING_NONE, PHASE>(L, a, b, c, d, wm, s, fork);
+    case 4: return launch_sweep<S, D2Q9_MRT, D2Q9_GUO_LINEAR, PHASE>(L, a, b, c, d, wm, s, fork);
+    case 5: return launch_sweep<S, D2Q9_MRT, D2Q9_GUO_SECOND_ORDER, PHASE>(L, a, b, c, d, wm, s, fork);
   }
   return fail(D2Q9_E_ARG, "bad collision/forcing");
 }
@@ -499,6 +581,73 @@ int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
   if (mom_src == mom_dst) return fail(D2Q9_E_ARG, "mom_src and mom_dst must differ");
   return dispatch<PHASE_FUSED>(L, f_src, f_dst, mom_src, mom_dst, write_moments, stream);
 }
+
+int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
+                     const d2q9_lattice* L_rim_b, const int32_t* deep_tiles,
+                     int32_t n_deep, const void* f_src, void* f_dst, void* f_tmp,
+                     const void* mom_src, void* mom_dst, void* mom_tmp,
+                     int write_moments, void* stream) {
+  for (const d2q9_lattice* X : {L, L_rim_a, L_rim_b})
+    if (int r = validate(X)) return r;
+  for (const d2q9_lattice* X : {L_rim_a, L_rim_b}) {
+    if (X->storage != L->storage || X->nx != L->nx || X->ny != L->ny ||
+        X->pitch != L->pitch || X->plane != L->plane || X->origin != L->origin ||
+        X->nbj != L->nbj || X->tile_i != L->tile_i || X->collision != L->collision ||
+        X->forcing != L->forcing || !X->tiles)
+      return fail(D2Q9_E_ARG, "rim descriptors must describe the same lattice, with tile lists");
+  }
+  if (L->storage != D2Q9_STORE_F64) return fail(D2Q9_E_ARG, "two-step sweep: FP64 storage only");
+  if (n_deep < 0 || (n_deep > 0 && !deep_tiles)) return fail(D2Q9_E_ARG, "bad deep tile list");
+  if (!f_src || !f_dst || !f_tmp || !mom_src || !mom_dst || !mom_tmp)
+    return fail(D2Q9_E_ARG, "null buffer");
+  if (f_src == f_dst || f_src == f_tmp || f_dst == f_tmp || mom_src == mom_dst ||
+      mom_src == mom_tmp || mom_dst == mom_tmp)
+    return fail(D2Q9_E_ARG, "f_src, f_dst, f_tmp (and the moments) must all differ");
+  const int ntj = (L->ny + 63) / 64;
+  if ((int64_t)n_deep > (int64_t)ntj * ((L->nx + DEEP_ROWS - 1) / DEEP_ROWS))
+    return fail(D2Q9_E_ARG, "more deep tiles than the lattice holds");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Aux* a = nullptr;
+  if (int r = aux_for_current_device(&a)) return r;
+  if (int r = check_hip(hipEventRecord(a->fork, s), "hipEventRecord fork")) return r;
+  if (int r = check_hip(hipStreamWaitEvent(a->stream, a->fork, 0), "hipStreamWaitEvent fork"))
+    return r;
+  // rim: step n + 1 of the rim tiles and the cells they pull from into
+  // f_tmp, then step n + 2 of the rim tiles into f_dst, on the aux stream
+  if (int r = dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(L_rim_a, f_src, f_tmp, mom_src,
+                                                          mom_tmp, 0, a->stream, false))
+    return r;
+  if (int r = dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(L_rim_b, f_tmp, f_dst, mom_tmp,
+                                                          mom_dst, write_moments, a->stream,
+                                                          false))
+    return r;
+  if (n_deep > 0) {
+    auto fs = static_cast<const double*>(f_src);
+    auto fd = static_cast<double*>(f_dst);
+    auto md = static_cast<double*>(mom_dst);
+    const dim3 grid((unsigned)n_deep), block(64, DEEP_WAVES);
+    switch (L->collision * 3 + L->forcing) {
+#define D2Q9_DEEP(KEY, M, F)                                                              \
+  case KEY:                                                                               \
+    hipLaunchKernelGGL((k_deep2<M, F>), grid, block, 0, s, *L, deep_tiles, n_deep, ntj, fs, \
+                       fd, md, write_moments);                                            \
+    break;
+      D2Q9_DEEP(0, D2Q9_BGK, D2Q9_FORCING_NONE)
+      D2Q9_DEEP(1, D2Q9_BGK, D2Q9_GUO_LINEAR)
+      D2Q9_DEEP(2, D2Q9_BGK, D2Q9_GUO_SECOND_ORDER)
+      D2Q9_DEEP(3, D2Q9_MRT, D2Q9_FORCING_NONE)
+      D2Q9_DEEP(4, D2Q9_MRT, D2Q9_GUO_LINEAR)
+      D2Q9_DEEP(5, D2Q9_MRT, D2Q9_GUO_SECOND_ORDER)
+#undef D2Q9_DEEP
+      default: return fail(D2Q9_E_ARG, "bad collision/forcing");
+    }
+    if (int r = check_hip(hipGetLastError(), "k_deep2 launch")) return r;
+  }
+  if (int r = check_hip(hipEventRecord(a->join, a->stream), "hipEventRecord join")) return r;
+  return check_hip(hipStreamWaitEvent(s, a->join, 0), "hipStreamWaitEvent join");
+}
+
+int d2q9_deep_rows(void) { return DEEP_ROWS; }
 
 int d2q9_moments(const d2q9_lattice* L, const void* f, void* mom, void* stream) {
   if (int r = validate(L)) return r;

@@ -29,6 +29,7 @@ reference's either way.
 """
 import ctypes
 import math
+from types import SimpleNamespace
 
 import numpy as np
 import torch
@@ -309,6 +310,119 @@ class CaseTables:
         return self.tile_class
 
 
+    def deep_split(self, rows):
+        """Deep tiles of the two-step sweep (d2q9_fused_step2) and the rim
+        tables around them.
+
+        A deep tile is ``rows`` x 64 interior cells (block-aligned, rows a
+        multiple of the tile rows) whose cells and neighbours one cell deep
+        are all plain fluid: no boundary rule, solid neighbour, stored
+        moment or periodic edge, and inside the lattice.  The rim is every
+        other cell, swept by whole tiles of the current tile shape (a tile
+        is either inside one deep tile or in the rim).  Rim B lists the rim
+        tiles with their fix-up cells; rim A lists the same tiles and adds
+        to the fix-up list every deep cell a rim cell pulls from, so that
+        its first step leaves in the scratch lattice all that the second
+        reads.  Returns a namespace (deep_ids, n_deep, tile_list, n_bulk,
+        n_general, bulk_masks, wall_index, fix_a, fix_b)."""
+        if self.tile_j != 64 or rows % self.tile_i:
+            raise ValueError("deep tiles need 64-cell blocks and rows that "
+                             "are a multiple of the tile rows")
+        nx, ny = self.nx, self.ny
+        inner = self.flags_pad[1:-1, 1:-1]
+        plain = inner == nt.CELL_FLUID
+        if self.x_periodic:
+            plain[0, :] = plain[-1, :] = False
+        if self.y_periodic:
+            plain[:, 0] = plain[:, -1] = False
+        sat = np.zeros((nx + 1, ny + 1), dtype=np.int64)
+        sat[1:, 1:] = (~plain).astype(np.int64).cumsum(0).cumsum(1)
+        ntj = -(-ny // 64)
+        nti2 = -(-nx // rows)
+        ti = np.arange(nti2)[:, None]
+        tj = np.arange(ntj)[None, :]
+        # grown region (0-based, inclusive): rows ti*rows - 1 .. ti*rows +
+        # rows, cells tj*64 - 1 .. tj*64 + 64
+        r0, r1 = ti * rows - 1, ti * rows + rows
+        c0, c1 = tj * 64 - 1, tj * 64 + 64
+        ok = (r0 >= 0) & (r1 <= nx - 1) & (c0 >= 0) & (c1 <= ny - 1)
+        r0c, r1c = np.clip(r0, 0, nx - 1), np.clip(r1, 0, nx - 1)
+        c0c, c1c = np.clip(c0, 0, ny - 1), np.clip(c1, 0, ny - 1)
+        bad = (sat[r1c + 1, c1c + 1] - sat[r0c, c1c + 1] - sat[r1c + 1, c0c]
+               + sat[r0c, c0c])
+        ok &= bad == 0
+        deep_ids = (ti * ntj + tj)[ok].astype(np.int32)
+        covered = np.repeat(np.repeat(ok, rows, axis=0), 64,
+                            axis=1)[:nx, :ny]
+        # rim tiles of the current tile shape
+        nti = self.nti
+        cpad = np.zeros((nti * self.tile_i, ntj * 64), dtype=bool)
+        cpad[:nx, :ny] = covered
+        in_deep = cpad.reshape(nti, self.tile_i, ntj, 64).all(axis=(1, 3))
+        in_deep = in_deep.ravel()
+        tl = np.asarray(self.tile_list)
+        bulk_pos = np.nonzero(~in_deep[tl[:self.n_bulk]])[0]
+        gen = tl[self.n_bulk:self.n_bulk + self.n_general]
+        gen = gen[~in_deep[gen]]
+        words = self.tile_j // 64
+        if self.n_bulk:
+            masks = self.bulk_masks.reshape(self.n_bulk, self.tile_i, words)
+            bulk_masks = np.ascontiguousarray(masks[bulk_pos]).reshape(-1)
+        else:
+            bulk_masks = np.zeros(0, dtype=np.uint64)
+        if bulk_masks.size == 0:
+            bulk_masks = np.zeros(1, dtype=np.uint64)
+        wall_index = (self.wall_index[bulk_pos].astype(np.int32)
+                      if self.wall_masks is not None else None)
+        pitch = int(math.ceil((ny + 2) / ALIGN) * ALIGN)
+
+        def cells_of(c):
+            c = np.asarray(c, dtype=np.int64) - ORIGIN
+            return c // pitch - 1, c % pitch - 1
+
+        fi, fj = cells_of(self.fix_cells[:self.n_fix])
+        keep = ~covered[fi, fj] if self.n_fix else np.zeros(0, dtype=bool)
+        fix_b = np.asarray(self.fix_cells[:self.n_fix])[keep].astype(np.int64)
+        # deep cells a rim fluid cell pulls from (periodic wrap included)
+        rim = ~covered & ((inner & nt.CELL_FLUID) != 0)
+        near = np.zeros_like(rim)
+        for di in (-1, 0, 1):
+            for dj in (-1, 0, 1):
+                sh = np.roll(rim, (di, dj), axis=(0, 1))
+                if not self.x_periodic:
+                    if di == 1:
+                        sh[0, :] = False
+                    elif di == -1:
+                        sh[-1, :] = False
+                if not self.y_periodic:
+                    if dj == 1:
+                        sh[:, 0] = False
+                    elif dj == -1:
+                        sh[:, -1] = False
+                near |= sh
+        ri, rj = np.nonzero(near & covered)
+        ring = ORIGIN + (ri + 1).astype(np.int64) * pitch + (rj + 1)
+        fix_a = np.union1d(fix_b, ring)
+
+        def as_list(c):
+            c = np.asarray(c, dtype=np.int32)
+            return (c, int(c.size)) if c.size else (np.zeros(1, np.int32), 0)
+
+        tile_list = np.concatenate([tl[:self.n_bulk][bulk_pos],
+                                    gen]).astype(np.int32)
+        if tile_list.size == 0:
+            tile_list = np.zeros(1, dtype=np.int32)
+        fa, nfa = as_list(fix_a)
+        fb, nfb = as_list(fix_b)
+        return SimpleNamespace(
+            deep_ids=deep_ids if deep_ids.size else np.zeros(1, np.int32),
+            n_deep=int(deep_ids.size), rows=int(rows), covered=covered,
+            tile_list=tile_list, n_bulk=int(bulk_pos.size),
+            n_general=int(gen.size), bulk_masks=bulk_masks,
+            wall_index=wall_index, fix_a=fa, n_fix_a=nfa, fix_b=fb,
+            n_fix_b=nfb)
+
+
 def force_tables(nx, ny, groups):
     """Device-ready node and group records for d2q9_momentum_exchange.
 
@@ -364,14 +478,18 @@ class D2Q9Engine:
     "MRT"), ``forcing`` (None, "guo_linear", "guo_second_order"),
     ``omega``, ``gravity``; plus the storage form ("f64", "f32", "f16dev",
     "bf16dev") and an optional per-cell ``force_field`` (padded
-    ``[size, 2]``) that replaces ``rho * gravity``.
+    ``[size, 2]``) that replaces ``rho * gravity``.  ``solid_tiles`` picks
+    how tiles holding solids are swept (see ``CaseTables.set_tile``);
+    ``two_step`` (default: on for FP64) advances pairs of steps with one
+    sweep over the deep interior (d2q9_fused_step2) when deep tiles fit.
     """
 
     def __init__(self, nx, ny, solid_pad, density, velocity, elements,
                  x_periodic, y_periodic, model, forcing, omega, gravity,
                  inv_cs_2, inv_cs_4, weights, float_min, storage="f64",
                  force_field=None, tile=(64, 4), rates=None, dev_scale=None,
-                 device=None, graph_steps=16, stream=None, solid_tiles=None):
+                 device=None, graph_steps=16, stream=None, solid_tiles=None,
+                 two_step=None):
         self.lib = nt.lib()
         self.device = (torch.device(device) if device is not None else
                        torch.device("cuda", torch.cuda.current_device()))
@@ -474,6 +592,11 @@ class D2Q9Engine:
         self._upload_tiles()
         self.graph_steps = max(2, int(graph_steps) // 2 * 2)
         self._graphs = {}
+        # FP64 advances pairs of steps with one sweep over the deep interior
+        self._two_step_req = (self.storage == nt.STORE_F64
+                              if two_step is None else bool(two_step))
+        self.two_step = False
+        self._setup_two_step()
         self.time_step = 0
         self._init_density = np.asarray(density, dtype=np.float64)
         self._init_velocity = np.asarray(velocity, dtype=np.float64)
@@ -507,6 +630,57 @@ class D2Q9Engine:
         L.wall_index = (self.wall_index.data_ptr()
                         if self.wall_index is not None else None)
 
+    def _setup_two_step(self):
+        """Deep tiles and rim descriptors of the two-step sweep, and its
+        scratch lattice; two-step stays off when no deep tile fits."""
+        self.two_step = False
+        if not self._two_step_req:
+            return
+        if self.storage != nt.STORE_F64:
+            raise ValueError("the two-step sweep is built for FP64 storage")
+        d = self.tables.deep_split(int(self.lib.d2q9_deep_rows()))
+        if d.n_deep == 0:
+            return
+        dev = self.device
+        with torch.cuda.stream(self.stream):
+            self._deep = {
+                "deep": torch.from_numpy(d.deep_ids).to(dev),
+                "tiles": torch.from_numpy(d.tile_list).to(dev),
+                "masks": torch.from_numpy(d.bulk_masks.view(np.int64)).to(dev),
+                "fix_a": torch.from_numpy(d.fix_a).to(dev),
+                "fix_b": torch.from_numpy(d.fix_b).to(dev),
+                "wall": (torch.from_numpy(d.wall_index).to(dev)
+                         if d.wall_index is not None else None),
+            }
+            if not hasattr(self, "f_tmp"):
+                # ghost and solid cells of the scratch lattice hold the
+                # encoded zero (0.0) like those of f[0] and f[1]
+                self.f_tmp = torch.zeros(self.pop_len, dtype=self.raw_dtype,
+                                         device=dev)
+                self.mom_tmp = torch.zeros(3 * self.plane, dtype=self.ctype,
+                                           device=dev)
+        t = self._deep
+        rims = []
+        for fix, nfix in ((t["fix_a"], d.n_fix_a), (t["fix_b"], d.n_fix_b)):
+            R = nt.Lattice.from_buffer_copy(self.L)
+            R.tiles = t["tiles"].data_ptr()
+            R.n_bulk, R.n_general = d.n_bulk, d.n_general
+            R.bulk_masks = t["masks"].data_ptr()
+            R.fix_cells = fix.data_ptr()
+            R.n_fix = nfix
+            R.wall_index = (t["wall"].data_ptr() if t["wall"] is not None
+                            else None)
+            rims.append(R)
+        self._rim_a, self._rim_b = rims
+        self._rim_ap = ctypes.byref(self._rim_a)
+        self._rim_bp = ctypes.byref(self._rim_b)
+        self.n_deep = d.n_deep
+        self.deep_fraction = float(d.covered.mean())
+        self.two_step = True
+        # a graph replays an even number of two-step sweeps, so the buffers
+        # return to their roles
+        self.graph_steps = max(4, self.graph_steps // 4 * 4)
+
     def set_tile_rows(self, tile_i):
         """Re-tile with ``tile_i`` rows per tile (workgroup 64 x tile_i);
         drops captured graphs, which hold the old tile tables."""
@@ -518,6 +692,8 @@ class D2Q9Engine:
         self.tables.set_tile(self.B, tile_i, self.tables.solid_tiles_req)
         self.tile_i = tile_i
         self._upload_tiles()
+        if self.two_step:
+            self._setup_two_step()
 
     # ---- layout helpers -------------------------------------------------
     def _upload_plane(self, pad, dtype, count=1):
@@ -591,6 +767,30 @@ class D2Q9Engine:
             int(write_mom), self._stream_ptr), "d2q9_fused_step")
         self.i_mom = 1 - self.i_mom
 
+    def _fused2(self, src, dst, write_mom):
+        nt.check(self.lib.d2q9_fused_step2(
+            self._Lp, self._rim_ap, self._rim_bp,
+            nt.ptr(self._deep["deep"]), self.n_deep,
+            nt.ptr(self.f[src]), nt.ptr(self.f[dst]), nt.ptr(self.f_tmp),
+            nt.ptr(self.mom[self.i_mom]), nt.ptr(self.mom[1 - self.i_mom]),
+            nt.ptr(self.mom_tmp), int(write_mom), self._stream_ptr),
+            "d2q9_fused_step2")
+        self.i_mom = 1 - self.i_mom
+
+    def step2(self, write_moments=False):
+        """Advance two reference time steps with the two-step sweep."""
+        if self.i_pop_new is not None:
+            self.step()
+            self.step(write_moments)
+            return
+        src = self.i_pop
+        dst = 1 - src
+        self._fused2(src, dst, write_moments)
+        self.i_pop = dst
+        self.i_pop_new = None
+        self.mom_full = bool(write_moments)
+        self.time_step += 2
+
     def step(self, write_moments=False):
         """Advance one reference time step."""
         if self.i_pop_new is not None:
@@ -617,6 +817,19 @@ class D2Q9Engine:
         if self.i_pop_new is not None:
             self.step(write_moments and n == 1)
             n -= 1
+        if self.two_step:
+            # pairs of steps: graphs of graph_steps steps, the last pair
+            # eager so that it can store the moments
+            if use_graph:
+                while n - 2 >= self.graph_steps:
+                    g = self._graph_for_state()
+                    nt.check(self.lib.d2q9_graph_launch(g, self._stream_ptr),
+                             "graph launch")
+                    self.time_step += self.graph_steps
+                    n -= self.graph_steps
+            while n >= 2:
+                self.step2(write_moments and n == 2)
+                n -= 2
         # full chunks of fused steps replay a captured graph; the last step
         # stays eager so that it can store the moments
         if use_graph:
@@ -638,7 +851,8 @@ class D2Q9Engine:
             g = self._graph_capture(self.i_pop, self.i_mom)
             self.i_mom = saved_mom
             self._graphs[key] = g
-        # graph_steps is even: buffers return to the same roles
+        # graph_steps is even (a multiple of 4 for two-step sweeps): the
+        # buffers return to the same roles
         self.mom_full = False
         return g
 
@@ -646,11 +860,21 @@ class D2Q9Engine:
         nt.check(self.lib.d2q9_graph_begin(self._stream_ptr), "graph begin")
         err = None
         src = start
-        for _ in range(self.graph_steps):
-            rc = self.lib.d2q9_fused_step(
-                self._Lp, nt.ptr(self.f[src]), nt.ptr(self.f[1 - src]),
-                nt.ptr(self.mom[im]), nt.ptr(self.mom[1 - im]), 0,
-                self._stream_ptr)
+        for _ in range(self.graph_steps // 2 if self.two_step
+                       else self.graph_steps):
+            if self.two_step:
+                rc = self.lib.d2q9_fused_step2(
+                    self._Lp, self._rim_ap, self._rim_bp,
+                    nt.ptr(self._deep["deep"]), self.n_deep,
+                    nt.ptr(self.f[src]), nt.ptr(self.f[1 - src]),
+                    nt.ptr(self.f_tmp), nt.ptr(self.mom[im]),
+                    nt.ptr(self.mom[1 - im]), nt.ptr(self.mom_tmp), 0,
+                    self._stream_ptr)
+            else:
+                rc = self.lib.d2q9_fused_step(
+                    self._Lp, nt.ptr(self.f[src]), nt.ptr(self.f[1 - src]),
+                    nt.ptr(self.mom[im]), nt.ptr(self.mom[1 - im]), 0,
+                    self._stream_ptr)
             if rc != 0 and err is None:
                 err = self.lib.d2q9_last_error().decode()
             src, im = 1 - src, 1 - im

@@ -63,14 +63,16 @@ class Solver:
     as in the reference.  Extra keywords select the population storage
     (``"f64"`` is the reference-exact path), the launch tile, the HIP
     graph chunk, the torch stream the engine runs on (a new one if None)
-    and how tiles holding solid cells are swept (``solid_tiles``: "sweep",
-    "cells" or "auto"; None picks per storage form).
+    how tiles holding solid cells are swept (``solid_tiles``: "sweep",
+    "cells" or "auto"; None picks per storage form) and whether FP64 pairs
+    of steps run as one sweep over the deep interior (``two_step``; None:
+    on for FP64).  Every choice gives bitwise the same results.
     """
 
     def __init__(self, comm=None, backend="gpu", n_threads=None,
                  simulation=None, storage="f64", tile=(64, 4),
                  graph_steps=16, verbose=True, output_root=".", stream=None,
-                 solid_tiles=None):
+                 solid_tiles=None, two_step=None):
         comm = comm if comm is not None else SingleRankComm()
         mpi_rank = comm.Get_rank()
         self.verbose = verbose
@@ -83,7 +85,7 @@ class Solver:
         self.backend = Backend(comm, self.state, backend, n_threads,
                                storage=storage, tile=tile,
                                graph_steps=graph_steps, stream=stream,
-                               solid_tiles=solid_tiles)
+                               solid_tiles=solid_tiles, two_step=two_step)
         self.mpi_operator = MPIOperator(comm, self.state)
         self.collision_operator = CollisionOperator(simulation, self.model,
                                                     self.state, comm,

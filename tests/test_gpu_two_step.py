@@ -1,0 +1,97 @@
+"""FP64 pairs of steps as one sweep over the deep interior
+(d2q9_fused_step2) against the oracle and against single steps.
+
+Bar: max relative error 1e-12 against the oracle (expected bitwise), and
+bitwise equality with the single-step path, after odd and even step counts,
+with and without graph replay, with the moments stored at the last step.
+"""
+import numpy as np
+import pytest
+
+from oracle import d2q9_oracle as O
+from pylabolt_amd import cases
+from tests.helpers import max_rel, oracle_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _rough(sim):
+    sim.initial_fields_dict["default"]["fluid"]["velocity"] = {
+        "type": "func",
+        "func": lambda i, j: (0.03 * np.sin(1.3 * i + 2.1 * j),
+                              0.03 * np.cos(0.7 * i - 1.9 * j))}
+    return sim
+
+
+def _forced_channel():
+    sim = cases.poiseuille_channel(80, 250, model="MRT",
+                                   forcing="guo_second_order")
+    sim.forcing_dict["gravity"] = [2e-5, 3e-6]
+    return sim
+
+
+CASES = {
+    "cylinder": lambda: cases.cylinder(120, 300, diameter=20, xc=40),
+    "cavity": lambda: cases.lid_driven_cavity(100, 200),
+    "channel_mrt_guo2": _forced_channel,
+    "cylinder_mrt": lambda: cases.cylinder(90, 260, diameter=16, xc=30,
+                                           model="MRT"),
+}
+
+
+def _solver(sim, two_step, graph_steps=8):
+    from pylabolt_amd.solvers.fluidLB import Solver
+    s = Solver(simulation=sim, storage="f64", graph_steps=graph_steps,
+               verbose=False, two_step=two_step)
+    s.set_backend()
+    return s
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_two_step_matches_oracle(name):
+    sim = _rough(CASES[name]())
+    case = oracle_case(sim)
+    O.initialize_pop(case)
+    s = _solver(sim, True)
+    eng = s.engine
+    assert eng.two_step and eng.n_deep > 0
+    print(f"[two-step] {name}: {eng.n_deep} deep tiles, "
+          f"{eng.deep_fraction:.0%} of the cells", flush=True)
+    done = 0
+    for chunk in (3, 2, 21):
+        s.advance(chunk)
+        O.run(case, chunk)
+        done += chunk
+        fl = case.fluid
+        rho, vel = eng.moments_host()
+        errs = {"pop": max_rel(eng.pop()[fl], case.pop[fl]),
+                "pop_new": max_rel(eng.pop_new()[fl], case.pop_new[fl]),
+                "density": max_rel(rho[fl], case.density[fl]),
+                "velocity": max_rel(vel[fl], case.velocity[fl])}
+        print(f"[two-step] {name} step {done}: {errs}", flush=True)
+        for k, v in errs.items():
+            assert v <= 1e-12, f"{name} step {done}: {k} {v}"
+
+
+@pytest.mark.parametrize("name", ["cylinder", "cavity"])
+def test_two_step_bitwise_equals_single_steps(name):
+    one = _solver(CASES[name](), False)
+    two = _solver(CASES[name](), True)
+    assert two.engine.two_step and not one.engine.two_step
+    done = 0
+    for chunk, graph in ((1, True), (2, False), (37, True), (18, False)):
+        one.engine.advance(chunk, use_graph=graph)
+        two.engine.advance(chunk, use_graph=graph)
+        done += chunk
+        assert np.array_equal(one.engine.pop(), two.engine.pop()), done
+        ra, ua = one.engine.moments_host()
+        rb, ub = two.engine.moments_host()
+        assert np.array_equal(ra, rb) and np.array_equal(ua, ub), done
+        print(f"[two-step] {name} bitwise equal after {done} steps",
+              flush=True)
+
+
+def test_two_step_off_without_deep_tiles():
+    s = _solver(cases.lid_driven_cavity(20, 30), True)
+    assert not s.engine.two_step
+    s.advance(5)

@@ -175,3 +175,82 @@ def test_solid_tiles_as_fix_cells(B):
     assert auto.solid_tiles == want
     with pytest.raises(ValueError):
         E.CaseTables(*args, solid_tiles="tiles")
+
+
+DEEP_CASES = {
+    "cylinder": lambda: cases.cylinder(120, 300, diameter=20, xc=40),
+    "cavity": lambda: cases.lid_driven_cavity(100, 200),
+    "channel": lambda: cases.poiseuille_channel(80, 250, model="BGK"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(DEEP_CASES))
+def test_deep_split_covers_and_feeds_the_rim(name):
+    """Deep tiles of the two-step sweep: block-aligned, disjoint, plain fluid
+    one cell around; every other cell lies in a listed rim tile; the first
+    rim step (list A) computes every cell the second (list B) pulls from."""
+    from pylabolt_amd import _native as nt
+    st = State(DEEP_CASES[name]())
+    nx, ny = (int(v) for v in st.mesh.grid_global_shape)
+    t = E.CaseTables(nx, ny, st.fields.solid,
+                     E.elements_from_boundary(st.boundary),
+                     st.boundary.x_periodic, st.boundary.y_periodic, 64, 4)
+    rows = 12
+    d = t.deep_split(rows)
+    assert d.n_deep > 0
+    inner = t.flags_pad[1:-1, 1:-1]
+    fluid = (inner & nt.CELL_FLUID) != 0
+    plain = inner == nt.CELL_FLUID
+    if t.x_periodic:
+        plain[[0, -1], :] = False
+    if t.y_periodic:
+        plain[:, [0, -1]] = False
+    ntj = -(-ny // 64)
+    cov = np.zeros((nx, ny), dtype=int)
+    for tid in d.deep_ids[:d.n_deep]:
+        ti, tj = divmod(int(tid), ntj)
+        a, b = ti * rows, tj * 64
+        assert a + rows <= nx - 1 and a >= 1 and b >= 1 and b + 64 <= ny - 1
+        assert plain[a - 1:a + rows + 1, b - 1:b + 65].all()
+        cov[a:a + rows, b:b + 64] += 1
+    assert cov.max() == 1
+    assert np.array_equal(cov == 1, d.covered)
+    # rim tiles: exactly the tiles not inside a deep tile
+    listed = np.zeros(t.nti * ntj, dtype=int)
+    for tid in d.tile_list[:d.n_bulk + d.n_general]:
+        listed[int(tid)] += 1
+    cpad = np.zeros((t.nti * 4, ntj * 64), dtype=bool)
+    cpad[:nx, :ny] = d.covered
+    inside = cpad.reshape(t.nti, 4, ntj, 64).all(axis=(1, 3)).ravel()
+    assert np.array_equal(listed, (~inside).astype(int))
+    # fix-up lists: B = the full list outside the deep tiles; A adds the deep
+    # cells that rim cells pull from
+    pitch, _ = E.geometry(nx, ny)
+
+    def ij(c):
+        c = np.asarray(c, dtype=np.int64) - E.ORIGIN
+        return c // pitch - 1, c % pitch - 1
+
+    full = set(int(c) for c in t.fix_cells[:t.n_fix])
+    fb = set(int(c) for c in d.fix_b[:d.n_fix_b])
+    fi, fj = ij(sorted(full))
+    assert fb == set(c for c, i, j in zip(sorted(full), fi, fj)
+                     if not d.covered[i, j])
+    fa = set(int(c) for c in d.fix_a[:d.n_fix_a])
+    assert fb <= fa
+    ai, aj = ij(sorted(fa - fb))
+    assert d.covered[ai, aj].all()
+    cx = [0, 1, 0, -1, 0, 1, -1, -1, 1]
+    cy = [0, 0, 1, 0, -1, 1, 1, -1, -1]
+    ri, rj = np.nonzero(fluid & ~d.covered)
+    for k in range(1, 9):
+        si, sj = ri - cx[k], rj - cy[k]
+        if t.x_periodic:
+            si %= nx
+        if t.y_periodic:
+            sj %= ny
+        ok = (si >= 0) & (si < nx) & (sj >= 0) & (sj < ny)
+        si, sj = si[ok], sj[ok]
+        src_deep = d.covered[si, sj]
+        c = E.ORIGIN + (si[src_deep] + 1) * pitch + (sj[src_deep] + 1)
+        assert set(c.tolist()) <= fa, k
