@@ -10,8 +10,13 @@ collision) over the whole lattice: one fused kernel launch, replayed from a
 HIP graph.  MLUPS = Nx * Ny * steps / t / 1e6.  The kernel time comes from
 HIP events recorded on the engine's own stream around the timed steps.
 
-Roofline: algorithmic traffic per cell update is 18 populations (9 read,
-9 written): 144 B in FP64, 72 B in FP32, 36 B for 16-bit storage.  Peak is
+Roofline: a single-step sweep reads and writes the 9 populations of every
+cell once per update: 144 B in FP64, 72 B in FP32, 36 B for 16-bit
+storage.  FP64 advances pairs of steps with one launch (d2q9_fused_step2):
+the deep interior is read and written once per two updates (144 B per cell
+and launch), the rim twice (288 B); ``achieved`` is those bytes over the
+launch's time (two steps), ``single_step_equivalent_gbs`` what a
+single-step sweep would have to move at the measured update rate.  Peak is
 MI355X's 8 TB/s HBM3E.  ``traffic`` is the measured HBM bytes per launch
 from rocprofv3 FETCH_SIZE/WRITE_SIZE counters (profiles/), when present.
 
@@ -318,7 +323,21 @@ def run(real_stdout):
     ms_step = ms / args.steps
     mlups = cells / (ms_step * 1e-3) / 1e6
     bpc = BYTES_PER_CELL[args.storage]
-    achieved = bpc * cells / (ms_step * 1e-3) / 1e9
+    single_equiv = bpc * cells / (ms_step * 1e-3) / 1e9
+    achieved = single_equiv
+    eng = solver.engine
+    two = None
+    if getattr(eng, "two_step", False):
+        deep = int(round(eng.deep_fraction * cells))
+        rim = cells - deep
+        launch_bytes = bpc * deep + 2 * bpc * rim
+        achieved = launch_bytes / (2 * ms_step * 1e-3) / 1e9
+        two = {"launch": "d2q9_fused_step2 (k_deep2 + rim sweeps), two "
+                         "steps per launch",
+               "deep_cells": deep, "rim_cells": rim,
+               "deep_fraction": deep / cells,
+               "algorithmic_bytes_per_launch": launch_bytes,
+               "launch_us": 2 * ms_step * 1e3}
     log(f"{args.storage}: {ms_step * 1e3:.1f} us/step  {mlups:.0f} MLUPS  "
         f"{achieved:.0f} GB/s ({achieved / PEAK_GBS:.3f} of peak), wall "
         f"{wall:.2f}s")
@@ -375,6 +394,8 @@ def run(real_stdout):
         log(f"copy kernel: {copy_gbs:.0f} GB/s")
     pmc = load_pmc(args.workload, args.storage)
     traffic = pmc["hbm_bytes_per_launch"] if pmc else None
+    if pmc and two is not None and pmc.get("launch") != "two-step":
+        traffic = None     # counters of another launch shape
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -401,13 +422,19 @@ def run(real_stdout):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_GBS,
                      "traffic": traffic,
-                     "bytes_per_cell_update": bpc},
+                     "bytes_per_cell_update": (
+                         two["algorithmic_bytes_per_launch"] / (2 * cells)
+                         if two else bpc),
+                     "single_step_bytes_per_cell_update": bpc,
+                     "single_step_equivalent_gbs": single_equiv},
         "cpu_baseline": cpu,
         "forms": forms,
         "batched": bat,
         "adjoint": adj,
         "copy_kernel_gbs": copy_gbs,
     }
+    if two is not None:
+        out["roofline"].update(two)
     if copy_gbs:
         out["roofline"]["frac_of_copy"] = achieved / copy_gbs
     return out
