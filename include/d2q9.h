@@ -208,13 +208,13 @@ int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
                     const void* mom_src, void* mom_dst, int write_moments,
                     void* stream);
 
-/* Two fused steps (FP64 storage): f_dst and mom_dst as after
+/* Two fused steps (FP64 or FP32 storage): f_dst and mom_dst as after
  * d2q9_fused_step(f_src -> f_tmp) and d2q9_fused_step(f_tmp -> f_dst),
  * bitwise.  It has no counterpart in the reference: it replaces two of its
  * time steps (solvers/fluidLB.py:206-253) by one sweep over the interior.
- * deep_tiles lists tiles of d2q9_deep_rows() rows x 64 cells (tile t: rows
- * 1 + (t / ntj) * rows .., cells 1 + (t % ntj) * 64 .., ntj =
- * ceil(ny / 64)) whose cells and whose neighbours one cell deep are plain
+ * deep_tiles lists tiles of d2q9_deep_rows() rows x tile_j cells (tile t:
+ * rows 1 + (t / ntj) * rows .., cells 1 + (t % ntj) * tile_j .., ntj =
+ * ceil(ny / tile_j)) whose cells and whose neighbours one cell deep are plain
  * fluid (flag D2Q9_CELL_FLUID only, no periodic edge); the caller
  * guarantees it.  Each is swept once for both steps, step n + 1 of the tile
  * grown by one cell held in LDS.  L_rim_a and L_rim_b describe the same

@@ -103,81 +103,103 @@ __global__ __launch_bounds__(256) void k_step_all(
   }
 }
 
-// Two fused steps of an FP64 deep tile: DEEP_ROWS rows x 64 cells whose
-// neighbourhood two cells deep holds only plain fluid cells (the host
-// checks it).  Stage one computes step n + 1 (pull from the lattice,
-// moments, collision) on the tile grown by one cell on every side into LDS;
-// stage two pulls step n + 1 from LDS and stores step n + 2.  Per cell and
-// two steps the lattice is read about 1.2 times (the grown rows and
-// columns) and written once, against twice each for two single steps.
-// Waves 0 .. DEEP_ROWS + 1 sweep the grown rows (lane l: cell j0 + l); the
-// last wave computes the grown columns j0 - 1 and j0 + 64 of every grown
-// row (lane 2r + side).  Arithmetic per cell is that of the single step.
+// Two fused steps of a deep tile: DEEP_ROWS rows x one block (B = 64 * CPL
+// cells) whose neighbourhood two cells deep holds only plain fluid cells
+// (the host checks it).  Stage one computes step n + 1 (pull from the
+// lattice, moments, collision) on the tile grown by one cell on every side
+// into LDS; stage two pulls step n + 1 from LDS and stores step n + 2.  Per
+// cell and two steps the lattice is read about 1.2 times (the grown rows
+// and columns, mostly L2 hits) and written once, against twice each for
+// two single steps.  Waves 0 .. DEEP_ROWS + 1 sweep the grown rows with the
+// single step's gather (lane l: cells j0 + CPL l ..); the last wave
+// computes the grown columns j0 - 1 and j0 + B of every grown row (lane
+// 2r + side).  Arithmetic per cell is that of the single step, so the
+// result is bitwise that of two single steps.  FP64 and FP32 storage.
 constexpr int DEEP_ROWS = 12;
 constexpr int DEEP_R1 = DEEP_ROWS + 2;
 constexpr int DEEP_WAVES = DEEP_R1 + 1;
 
-template <int MODEL, int FORCING>
-__device__ __forceinline__ void deep_collide(const d2q9_lattice& L, const Consts<double>& K,
-                                             double (&g)[9], int64_t c, double& rho,
-                                             double& ux, double& uy) {
-  double ffx = 0.0, ffy = 0.0;
+template <int S, int MODEL, int FORCING>
+__device__ __forceinline__ void deep_collide(const d2q9_lattice& L,
+                                             const Consts<typename Codec<S>::T>& K,
+                                             typename Codec<S>::T (&g)[9], int64_t c,
+                                             typename Codec<S>::T& rho,
+                                             typename Codec<S>::T& ux,
+                                             typename Codec<S>::T& uy) {
+  using T = typename Codec<S>::T;
+  T ffx = (T)0, ffy = (T)0;
   if (L.force_source == D2Q9_FSRC_FIELD) {
-    const double* ff = static_cast<const double*>(L.force_field);
+    const T* ff = static_cast<const T*>(L.force_field);
     ffx = ff[c];
     ffy = ff[L.plane + c];
   }
-  double drho, fx, fy;
-  moments<double, FORCING, false>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
-  collide<double, MODEL, FORCING, false>(g, K, drho, rho, ux, uy, fx, fy);
+  T drho, fx, fy;
+  moments<T, FORCING, Codec<S>::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
+  collide<T, MODEL, FORCING, Codec<S>::dev>(g, K, drho, rho, ux, uy, fx, fy);
 }
 
-template <int MODEL, int FORCING>
+template <int S, int MODEL, int FORCING>
 __global__ __launch_bounds__(64 * DEEP_WAVES) void k_deep2(
     const d2q9_lattice L, const int32_t* __restrict__ deep, int n_deep, int ntj,
-    const double* __restrict__ fsrc, double* __restrict__ fdst, double* __restrict__ mdst,
-    int write_mom) {
-  __shared__ double s1[DEEP_R1][9][66];
+    const typename Codec<S>::raw* __restrict__ fsrc, typename Codec<S>::raw* __restrict__ fdst,
+    typename Codec<S>::T* __restrict__ mdst, int write_mom) {
+  using C = Codec<S>;
+  using T = typename C::T;
+  using R = typename C::raw;
+  constexpr int CPL = C::CPL;
+  constexpr int B = 64 * CPL;
+  __shared__ T s1[DEEP_R1][9][B + 2];
   const int lane = (int)threadIdx.x, w = (int)threadIdx.y;
   const int t = deep[xcd_remap((int)blockIdx.x, n_deep)];
-  const int i0 = 1 + (t / ntj) * DEEP_ROWS, j0 = 1 + (t % ntj) * 64;
-  const Consts<double> K = load_consts<double>(L);
-  {
-    int r = w, cl = lane + 1;
-    bool act = true;
-    if (w == DEEP_R1) {
-      r = lane >> 1;
-      cl = (lane & 1) ? 65 : 0;
-      act = lane < 2 * DEEP_R1;
-    }
-    if (act) {
-      const int i = i0 - 1 + r, j = j0 - 1 + cl;
-      double g[9];
+  const int tj = t % ntj;
+  const int i0 = 1 + (t / ntj) * DEEP_ROWS, j0 = 1 + tj * B;
+  const Consts<T> K = load_consts<T>(L);
+  if (w < DEEP_R1) {
+    // grown row i0 - 1 + w, the block's B cells
+    const int i = i0 - 1 + w, jl = j0 + lane * CPL;
+    T g[CPL][9];
+    gather_block<S, PHASE_FUSED>(L, K, fsrc, i, tj, lane, jl, g);
+    const int64_t c = cell_index(L, i, jl);
 #pragma unroll
-      for (int k = 0; k < 9; ++k) g[k] = fsrc[pop_index<64>(L, i - CX[k], j - CY[k], k)];
-      double rho, ux, uy;
-      deep_collide<MODEL, FORCING>(L, K, g, cell_index(L, i, j), rho, ux, uy);
+    for (int q = 0; q < CPL; ++q) {
+      T rho, ux, uy;
+      deep_collide<S, MODEL, FORCING>(L, K, g[q], c + q, rho, ux, uy);
 #pragma unroll
-      for (int k = 0; k < 9; ++k) s1[r][k][cl] = g[k];
+      for (int k = 0; k < 9; ++k) s1[w][k][1 + lane * CPL + q] = g[q][k];
     }
+  } else if (lane < 2 * DEEP_R1) {
+    // grown columns j0 - 1 and j0 + B
+    const int r = lane >> 1, cl = (lane & 1) ? B + 1 : 0;
+    const int i = i0 - 1 + r, j = j0 - 1 + cl;
+    T g[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g[k] = ld<S>(L, K, fsrc, i - CX[k], j - CY[k], k);
+    T rho, ux, uy;
+    deep_collide<S, MODEL, FORCING>(L, K, g, cell_index(L, i, j), rho, ux, uy);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s1[r][k][cl] = g[k];
   }
   __syncthreads();
   if (w >= 1 && w <= DEEP_ROWS) {
-    const int i = i0 - 1 + w, j = j0 + lane;
-    double g[9];
+    const int i = i0 - 1 + w, jl = j0 + lane * CPL;
+    const int64_t c = cell_index(L, i, jl);
+    R out[9][CPL];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) g[k] = s1[w - CX[k]][k][lane + 1 - CY[k]];
-    const int64_t c = cell_index(L, i, j);
-    double rho, ux, uy;
-    deep_collide<MODEL, FORCING>(L, K, g, c, rho, ux, uy);
-    const int64_t o = pop_index<64>(L, i, j, 0);
+    for (int q = 0; q < CPL; ++q) {
+      T g[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) store_nt(g[k], fdst + o + (int64_t)k * 64);
-    if (write_mom) {
-      mdst[c] = rho;
-      mdst[L.plane + c] = ux;
-      mdst[2 * L.plane + c] = uy;
+      for (int k = 0; k < 9; ++k) g[k] = s1[w - CX[k]][k][1 + lane * CPL + q - CY[k]];
+      T rho, ux, uy;
+      deep_collide<S, MODEL, FORCING>(L, K, g, c + q, rho, ux, uy);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) out[k][q] = C::enc(g[k], k, K);
+      if (write_mom) {
+        mdst[c + q] = rho;
+        mdst[L.plane + c + q] = ux;
+        mdst[2 * L.plane + c + q] = uy;
+      }
     }
+    store_block<S>(L, fdst, i, tj, lane, out);
   }
 }
 
@@ -596,14 +618,15 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
         X->forcing != L->forcing || !X->tiles)
       return fail(D2Q9_E_ARG, "rim descriptors must describe the same lattice, with tile lists");
   }
-  if (L->storage != D2Q9_STORE_F64) return fail(D2Q9_E_ARG, "two-step sweep: FP64 storage only");
+  if (L->storage != D2Q9_STORE_F64 && L->storage != D2Q9_STORE_F32)
+    return fail(D2Q9_E_ARG, "two-step sweep: FP64 or FP32 storage");
   if (n_deep < 0 || (n_deep > 0 && !deep_tiles)) return fail(D2Q9_E_ARG, "bad deep tile list");
   if (!f_src || !f_dst || !f_tmp || !mom_src || !mom_dst || !mom_tmp)
     return fail(D2Q9_E_ARG, "null buffer");
   if (f_src == f_dst || f_src == f_tmp || f_dst == f_tmp || mom_src == mom_dst ||
       mom_src == mom_tmp || mom_dst == mom_tmp)
     return fail(D2Q9_E_ARG, "f_src, f_dst, f_tmp (and the moments) must all differ");
-  const int ntj = (L->ny + 63) / 64;
+  const int ntj = (L->ny + L->tile_j - 1) / L->tile_j;
   if ((int64_t)n_deep > (int64_t)ntj * ((L->nx + DEEP_ROWS - 1) / DEEP_ROWS))
     return fail(D2Q9_E_ARG, "more deep tiles than the lattice holds");
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -614,33 +637,46 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
     return r;
   // rim: step n + 1 of the rim tiles and the cells they pull from into
   // f_tmp, then step n + 2 of the rim tiles into f_dst, on the aux stream
-  if (int r = dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(L_rim_a, f_src, f_tmp, mom_src,
-                                                          mom_tmp, 0, a->stream, false))
+  const bool f64 = L->storage == D2Q9_STORE_F64;
+  if (int r = f64 ? dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(L_rim_a, f_src, f_tmp, mom_src,
+                                                                mom_tmp, 0, a->stream, false)
+                  : dispatch_model<D2Q9_STORE_F32, PHASE_FUSED>(L_rim_a, f_src, f_tmp, mom_src,
+                                                                mom_tmp, 0, a->stream, false))
     return r;
-  if (int r = dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(L_rim_b, f_tmp, f_dst, mom_tmp,
-                                                          mom_dst, write_moments, a->stream,
-                                                          false))
+  if (int r = f64 ? dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(L_rim_b, f_tmp, f_dst, mom_tmp,
+                                                                mom_dst, write_moments,
+                                                                a->stream, false)
+                  : dispatch_model<D2Q9_STORE_F32, PHASE_FUSED>(L_rim_b, f_tmp, f_dst, mom_tmp,
+                                                                mom_dst, write_moments,
+                                                                a->stream, false))
     return r;
   if (n_deep > 0) {
-    auto fs = static_cast<const double*>(f_src);
-    auto fd = static_cast<double*>(f_dst);
-    auto md = static_cast<double*>(mom_dst);
     const dim3 grid((unsigned)n_deep), block(64, DEEP_WAVES);
-    switch (L->collision * 3 + L->forcing) {
-#define D2Q9_DEEP(KEY, M, F)                                                              \
-  case KEY:                                                                               \
-    hipLaunchKernelGGL((k_deep2<M, F>), grid, block, 0, s, *L, deep_tiles, n_deep, ntj, fs, \
-                       fd, md, write_moments);                                            \
+    const int key = L->collision * 3 + L->forcing;
+#define D2Q9_DEEP(SV, KEY, M, F)                                                         \
+  case KEY:                                                                              \
+    hipLaunchKernelGGL((k_deep2<SV, M, F>), grid, block, 0, s, *L, deep_tiles, n_deep, ntj, \
+                       static_cast<const Codec<SV>::raw*>(f_src),                        \
+                       static_cast<Codec<SV>::raw*>(f_dst),                              \
+                       static_cast<Codec<SV>::T*>(mom_dst), write_moments);              \
     break;
-      D2Q9_DEEP(0, D2Q9_BGK, D2Q9_FORCING_NONE)
-      D2Q9_DEEP(1, D2Q9_BGK, D2Q9_GUO_LINEAR)
-      D2Q9_DEEP(2, D2Q9_BGK, D2Q9_GUO_SECOND_ORDER)
-      D2Q9_DEEP(3, D2Q9_MRT, D2Q9_FORCING_NONE)
-      D2Q9_DEEP(4, D2Q9_MRT, D2Q9_GUO_LINEAR)
-      D2Q9_DEEP(5, D2Q9_MRT, D2Q9_GUO_SECOND_ORDER)
-#undef D2Q9_DEEP
-      default: return fail(D2Q9_E_ARG, "bad collision/forcing");
+#define D2Q9_DEEP_ALL(SV)                                          \
+  switch (key) {                                                   \
+    D2Q9_DEEP(SV, 0, D2Q9_BGK, D2Q9_FORCING_NONE)                  \
+    D2Q9_DEEP(SV, 1, D2Q9_BGK, D2Q9_GUO_LINEAR)                    \
+    D2Q9_DEEP(SV, 2, D2Q9_BGK, D2Q9_GUO_SECOND_ORDER)              \
+    D2Q9_DEEP(SV, 3, D2Q9_MRT, D2Q9_FORCING_NONE)                  \
+    D2Q9_DEEP(SV, 4, D2Q9_MRT, D2Q9_GUO_LINEAR)                    \
+    D2Q9_DEEP(SV, 5, D2Q9_MRT, D2Q9_GUO_SECOND_ORDER)              \
+    default: return fail(D2Q9_E_ARG, "bad collision/forcing");     \
+  }
+    if (f64) {
+      D2Q9_DEEP_ALL(D2Q9_STORE_F64)
+    } else {
+      D2Q9_DEEP_ALL(D2Q9_STORE_F32)
     }
+#undef D2Q9_DEEP_ALL
+#undef D2Q9_DEEP
     if (int r = check_hip(hipGetLastError(), "k_deep2 launch")) return r;
   }
   if (int r = check_hip(hipEventRecord(a->join, a->stream), "hipEventRecord join")) return r;

@@ -314,8 +314,8 @@ class CaseTables:
         """Deep tiles of the two-step sweep (d2q9_fused_step2) and the rim
         tables around them.
 
-        A deep tile is ``rows`` x 64 interior cells (block-aligned, rows a
-        multiple of the tile rows) whose cells and neighbours one cell deep
+        A deep tile is ``rows`` x one block (``tile_j`` cells) of interior
+        cells (block-aligned, rows a multiple of the tile rows) whose cells and neighbours one cell deep
         are all plain fluid: no boundary rule, solid neighbour, stored
         moment or periodic edge, and inside the lattice.  The rim is every
         other cell, swept by whole tiles of the current tile shape (a tile
@@ -325,9 +325,10 @@ class CaseTables:
         its first step leaves in the scratch lattice all that the second
         reads.  Returns a namespace (deep_ids, n_deep, tile_list, n_bulk,
         n_general, bulk_masks, wall_index, fix_a, fix_b)."""
-        if self.tile_j != 64 or rows % self.tile_i:
-            raise ValueError("deep tiles need 64-cell blocks and rows that "
-                             "are a multiple of the tile rows")
+        B = self.tile_j
+        if rows % self.tile_i:
+            raise ValueError("deep tile rows must be a multiple of the tile "
+                             "rows")
         nx, ny = self.nx, self.ny
         inner = self.flags_pad[1:-1, 1:-1]
         plain = inner == nt.CELL_FLUID
@@ -337,14 +338,14 @@ class CaseTables:
             plain[:, 0] = plain[:, -1] = False
         sat = np.zeros((nx + 1, ny + 1), dtype=np.int64)
         sat[1:, 1:] = (~plain).astype(np.int64).cumsum(0).cumsum(1)
-        ntj = -(-ny // 64)
+        ntj = -(-ny // B)
         nti2 = -(-nx // rows)
         ti = np.arange(nti2)[:, None]
         tj = np.arange(ntj)[None, :]
         # grown region (0-based, inclusive): rows ti*rows - 1 .. ti*rows +
-        # rows, cells tj*64 - 1 .. tj*64 + 64
+        # rows, cells tj*B - 1 .. tj*B + B
         r0, r1 = ti * rows - 1, ti * rows + rows
-        c0, c1 = tj * 64 - 1, tj * 64 + 64
+        c0, c1 = tj * B - 1, tj * B + B
         ok = (r0 >= 0) & (r1 <= nx - 1) & (c0 >= 0) & (c1 <= ny - 1)
         r0c, r1c = np.clip(r0, 0, nx - 1), np.clip(r1, 0, nx - 1)
         c0c, c1c = np.clip(c0, 0, ny - 1), np.clip(c1, 0, ny - 1)
@@ -352,13 +353,13 @@ class CaseTables:
                + sat[r0c, c0c])
         ok &= bad == 0
         deep_ids = (ti * ntj + tj)[ok].astype(np.int32)
-        covered = np.repeat(np.repeat(ok, rows, axis=0), 64,
+        covered = np.repeat(np.repeat(ok, rows, axis=0), B,
                             axis=1)[:nx, :ny]
         # rim tiles of the current tile shape
         nti = self.nti
-        cpad = np.zeros((nti * self.tile_i, ntj * 64), dtype=bool)
+        cpad = np.zeros((nti * self.tile_i, ntj * B), dtype=bool)
         cpad[:nx, :ny] = covered
-        in_deep = cpad.reshape(nti, self.tile_i, ntj, 64).all(axis=(1, 3))
+        in_deep = cpad.reshape(nti, self.tile_i, ntj, B).all(axis=(1, 3))
         in_deep = in_deep.ravel()
         tl = np.asarray(self.tile_list)
         bulk_pos = np.nonzero(~in_deep[tl[:self.n_bulk]])[0]
@@ -480,8 +481,9 @@ class D2Q9Engine:
     "bf16dev") and an optional per-cell ``force_field`` (padded
     ``[size, 2]``) that replaces ``rho * gravity``.  ``solid_tiles`` picks
     how tiles holding solids are swept (see ``CaseTables.set_tile``);
-    ``two_step`` (default: on for FP64) advances pairs of steps with one
-    sweep over the deep interior (d2q9_fused_step2) when deep tiles fit.
+    ``two_step`` (default: on for FP64 and FP32) advances pairs of steps
+    with one sweep over the deep interior (d2q9_fused_step2) when deep
+    tiles fit.
     """
 
     def __init__(self, nx, ny, solid_pad, density, velocity, elements,
@@ -592,8 +594,10 @@ class D2Q9Engine:
         self._upload_tiles()
         self.graph_steps = max(2, int(graph_steps) // 2 * 2)
         self._graphs = {}
-        # FP64 advances pairs of steps with one sweep over the deep interior
-        self._two_step_req = (self.storage == nt.STORE_F64
+        # FP64 and FP32 advance pairs of steps with one sweep over the deep
+        # interior (the 16-bit forms are bound by arithmetic, which the
+        # grown tiles would add to)
+        self._two_step_req = (self.storage in (nt.STORE_F64, nt.STORE_F32)
                               if two_step is None else bool(two_step))
         self.two_step = False
         self._setup_two_step()
@@ -636,8 +640,9 @@ class D2Q9Engine:
         self.two_step = False
         if not self._two_step_req:
             return
-        if self.storage != nt.STORE_F64:
-            raise ValueError("the two-step sweep is built for FP64 storage")
+        if self.storage not in (nt.STORE_F64, nt.STORE_F32):
+            raise ValueError("the two-step sweep is built for FP64 and FP32 "
+                             "storage")
         d = self.tables.deep_split(int(self.lib.d2q9_deep_rows()))
         if d.n_deep == 0:
             return
@@ -654,7 +659,8 @@ class D2Q9Engine:
             }
             if not hasattr(self, "f_tmp"):
                 # ghost and solid cells of the scratch lattice hold the
-                # encoded zero (0.0) like those of f[0] and f[1]
+                # encoded zero (0.0 in FP64 and FP32 storage) like those of
+                # f[0] and f[1]
                 self.f_tmp = torch.zeros(self.pop_len, dtype=self.raw_dtype,
                                          device=dev)
                 self.mom_tmp = torch.zeros(3 * self.plane, dtype=self.ctype,

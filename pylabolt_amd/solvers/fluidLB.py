@@ -64,9 +64,9 @@ class Solver:
     (``"f64"`` is the reference-exact path), the launch tile, the HIP
     graph chunk, the torch stream the engine runs on (a new one if None)
     how tiles holding solid cells are swept (``solid_tiles``: "sweep",
-    "cells" or "auto"; None picks per storage form) and whether FP64 pairs
-    of steps run as one sweep over the deep interior (``two_step``; None:
-    on for FP64).  Every choice gives bitwise the same results.
+    "cells" or "auto"; None picks per storage form) and whether pairs of
+    steps run as one sweep over the deep interior (``two_step``; None: on
+    for FP64 and FP32 storage).  Every choice gives bitwise the same results.
     """
 
     def __init__(self, comm=None, backend="gpu", n_threads=None,

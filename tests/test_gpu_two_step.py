@@ -1,9 +1,11 @@
-"""FP64 pairs of steps as one sweep over the deep interior
-(d2q9_fused_step2) against the oracle and against single steps.
+"""Pairs of steps as one sweep over the deep interior (d2q9_fused_step2)
+against the oracle (FP64) and against single steps (FP64 and FP32).
 
 Bar: max relative error 1e-12 against the oracle (expected bitwise), and
 bitwise equality with the single-step path, after odd and even step counts,
 with and without graph replay, with the moments stored at the last step.
+The lattices are wide enough (ny >= 2 blocks + 2) for deep tiles of both
+block widths (64 cells in FP64, 128 in FP32).
 """
 import numpy as np
 import pytest
@@ -24,7 +26,7 @@ def _rough(sim):
 
 
 def _forced_channel():
-    sim = cases.poiseuille_channel(80, 250, model="MRT",
+    sim = cases.poiseuille_channel(80, 300, model="MRT",
                                    forcing="guo_second_order")
     sim.forcing_dict["gravity"] = [2e-5, 3e-6]
     return sim
@@ -32,16 +34,16 @@ def _forced_channel():
 
 CASES = {
     "cylinder": lambda: cases.cylinder(120, 300, diameter=20, xc=40),
-    "cavity": lambda: cases.lid_driven_cavity(100, 200),
+    "cavity": lambda: cases.lid_driven_cavity(100, 300),
     "channel_mrt_guo2": _forced_channel,
     "cylinder_mrt": lambda: cases.cylinder(90, 260, diameter=16, xc=30,
                                            model="MRT"),
 }
 
 
-def _solver(sim, two_step, graph_steps=8):
+def _solver(sim, two_step, graph_steps=8, storage="f64"):
     from pylabolt_amd.solvers.fluidLB import Solver
-    s = Solver(simulation=sim, storage="f64", graph_steps=graph_steps,
+    s = Solver(simulation=sim, storage=storage, graph_steps=graph_steps,
                verbose=False, two_step=two_step)
     s.set_backend()
     return s
@@ -73,10 +75,11 @@ def test_two_step_matches_oracle(name):
             assert v <= 1e-12, f"{name} step {done}: {k} {v}"
 
 
-@pytest.mark.parametrize("name", ["cylinder", "cavity"])
-def test_two_step_bitwise_equals_single_steps(name):
-    one = _solver(CASES[name](), False)
-    two = _solver(CASES[name](), True)
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("name", ["cylinder", "cavity", "channel_mrt_guo2"])
+def test_two_step_bitwise_equals_single_steps(name, storage):
+    one = _solver(CASES[name](), False, storage=storage)
+    two = _solver(CASES[name](), True, storage=storage)
     assert two.engine.two_step and not one.engine.two_step
     done = 0
     for chunk, graph in ((1, True), (2, False), (37, True), (18, False)):
@@ -87,8 +90,8 @@ def test_two_step_bitwise_equals_single_steps(name):
         ra, ua = one.engine.moments_host()
         rb, ub = two.engine.moments_host()
         assert np.array_equal(ra, rb) and np.array_equal(ua, ub), done
-        print(f"[two-step] {name} bitwise equal after {done} steps",
-              flush=True)
+        print(f"[two-step] {name} {storage} bitwise equal after {done} "
+              f"steps", flush=True)
 
 
 def test_two_step_off_without_deep_tiles():
