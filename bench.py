@@ -12,11 +12,12 @@ HIP events recorded on the engine's own stream around the timed steps.
 
 Roofline: a single-step sweep reads and writes the 9 populations of every
 cell once per update: 144 B in FP64, 72 B in FP32, 36 B for 16-bit
-storage.  FP64 advances pairs of steps with one launch (d2q9_fused_step2):
-the deep interior is read and written once per two updates (144 B per cell
-and launch), the rim twice (288 B); ``achieved`` is those bytes over the
-launch's time (two steps), ``single_step_equivalent_gbs`` what a
-single-step sweep would have to move at the measured update rate.  Peak is
+storage.  FP64 and FP32 advance pairs of steps with one launch
+(d2q9_fused_step2): the deep interior is read and written once per two
+updates (144 B per cell and launch in FP64), the rim twice (288 B);
+``achieved`` is those bytes over the launch's time (two steps),
+``single_step_equivalent_gbs`` what a single-step sweep would have to move
+at the measured update rate.  Peak is
 MI355X's 8 TB/s HBM3E.  ``traffic`` is the measured HBM bytes per launch
 from rocprofv3 FETCH_SIZE/WRITE_SIZE counters (profiles/), when present.
 
@@ -246,6 +247,25 @@ def load_pmc(workload, storage):
     return None
 
 
+def sweep_rate(engine, bpc, ms_step, cells):
+    """HBM rate of the measured sweep: (achieved GB/s, single-step
+    equivalent GB/s, two-step details or None).  A two-step launch moves
+    bpc per deep cell and 2 bpc per rim cell for two updates."""
+    single = bpc * cells / (ms_step * 1e-3) / 1e9
+    if not getattr(engine, "two_step", False):
+        return single, single, None
+    deep = int(round(engine.deep_fraction * cells))
+    rim = cells - deep
+    launch_bytes = bpc * deep + 2 * bpc * rim
+    achieved = launch_bytes / (2 * ms_step * 1e-3) / 1e9
+    return achieved, single, {
+        "launch": "d2q9_fused_step2 (k_deep2 + rim sweeps), two steps per "
+                  "launch",
+        "deep_cells": deep, "rim_cells": rim, "deep_fraction": deep / cells,
+        "algorithmic_bytes_per_launch": launch_bytes,
+        "launch_us": 2 * ms_step * 1e3}
+
+
 def run_form(sim, storage, steps, warmup, tile, use_graph=True):
     from pylabolt_amd.solvers.fluidLB import Solver
     solver = Solver(simulation=sim, storage=storage, tile=tile,
@@ -323,21 +343,8 @@ def run(real_stdout):
     ms_step = ms / args.steps
     mlups = cells / (ms_step * 1e-3) / 1e6
     bpc = BYTES_PER_CELL[args.storage]
-    single_equiv = bpc * cells / (ms_step * 1e-3) / 1e9
-    achieved = single_equiv
-    eng = solver.engine
-    two = None
-    if getattr(eng, "two_step", False):
-        deep = int(round(eng.deep_fraction * cells))
-        rim = cells - deep
-        launch_bytes = bpc * deep + 2 * bpc * rim
-        achieved = launch_bytes / (2 * ms_step * 1e-3) / 1e9
-        two = {"launch": "d2q9_fused_step2 (k_deep2 + rim sweeps), two "
-                         "steps per launch",
-               "deep_cells": deep, "rim_cells": rim,
-               "deep_fraction": deep / cells,
-               "algorithmic_bytes_per_launch": launch_bytes,
-               "launch_us": 2 * ms_step * 1e3}
+    achieved, single_equiv, two = sweep_rate(solver.engine, bpc, ms_step,
+                                             cells)
     log(f"{args.storage}: {ms_step * 1e3:.1f} us/step  {mlups:.0f} MLUPS  "
         f"{achieved:.0f} GB/s ({achieved / PEAK_GBS:.3f} of peak), wall "
         f"{wall:.2f}s")
@@ -355,7 +362,8 @@ def run(real_stdout):
         s2, ms2, _ = run_form(sim, form, args.steps, args.warmup, tile)
         st2 = ms2 / args.steps
         m2 = cells / (st2 * 1e-3) / 1e6
-        a2 = BYTES_PER_CELL[form] * cells / (st2 * 1e-3) / 1e9
+        a2, single2, two2 = sweep_rate(s2.engine, BYTES_PER_CELL[form], st2,
+                                       cells)
         s2.engine.advance(steps_done - s2.engine.time_step, write_moments=True)
         _, vel2 = s2.engine.moments_host()
         du = vel2[fluid] - ref_vel[fluid]
@@ -364,6 +372,8 @@ def run(real_stdout):
         rms_ref = float(np.sqrt((du ** 2).sum(axis=-1).mean())) / U_REF
         forms[form] = {"mlups": m2, "us_per_step": st2 * 1e3, "gbs": a2,
                        "frac": a2 / PEAK_GBS,
+                       "two_step": two2 is not None,
+                       "single_step_equivalent_gbs": single2,
                        "speedup_vs_" + args.storage: m2 / mlups,
                        "u_rel_l2_vs_" + args.storage: l2,
                        "u_rms_err_over_u_ref": rms_ref,

@@ -113,8 +113,10 @@ __global__ __launch_bounds__(256) void k_step_all(
 // two single steps.  Waves 0 .. DEEP_ROWS + 1 sweep the grown rows with the
 // single step's gather (lane l: cells j0 + CPL l ..); the last wave
 // computes the grown columns j0 - 1 and j0 + B of every grown row (lane
-// 2r + side).  Arithmetic per cell is that of the single step, so the
-// result is bitwise that of two single steps.  FP64 and FP32 storage.
+// 2r + side).  Arithmetic per cell is that of the single step, so in FP64
+// and FP32 storage the result is bitwise that of two single steps; in the
+// 16-bit forms the intermediate step stays in FP32 (LDS) instead of being
+// rounded to storage, and only the stored step is rounded.
 constexpr int DEEP_ROWS = 12;
 constexpr int DEEP_R1 = DEEP_ROWS + 2;
 constexpr int DEEP_WAVES = DEEP_R1 + 1;
@@ -191,8 +193,14 @@ __global__ __launch_bounds__(64 * DEEP_WAVES) void k_deep2(
       for (int k = 0; k < 9; ++k) g[k] = s1[w - CX[k]][k][1 + lane * CPL + q - CY[k]];
       T rho, ux, uy;
       deep_collide<S, MODEL, FORCING>(L, K, g, c + q, rho, ux, uy);
+      if constexpr (C::dev && sizeof(R) == 2) {
+        const uint32_t key = cell_key((uint32_t)(c + q));
 #pragma unroll
-      for (int k = 0; k < 9; ++k) out[k][q] = C::enc(g[k], k, K);
+        for (int k = 0; k < 9; ++k) out[k][q] = C::encs(g[k], k, key, K);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) out[k][q] = C::enc(g[k], k, K);
+      }
       if (write_mom) {
         mdst[c + q] = rho;
         mdst[L.plane + c + q] = ux;
@@ -618,8 +626,7 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
         X->forcing != L->forcing || !X->tiles)
       return fail(D2Q9_E_ARG, "rim descriptors must describe the same lattice, with tile lists");
   }
-  if (L->storage != D2Q9_STORE_F64 && L->storage != D2Q9_STORE_F32)
-    return fail(D2Q9_E_ARG, "two-step sweep: FP64 or FP32 storage");
+
   if (n_deep < 0 || (n_deep > 0 && !deep_tiles)) return fail(D2Q9_E_ARG, "bad deep tile list");
   if (!f_src || !f_dst || !f_tmp || !mom_src || !mom_dst || !mom_tmp)
     return fail(D2Q9_E_ARG, "null buffer");
@@ -637,19 +644,17 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
     return r;
   // rim: step n + 1 of the rim tiles and the cells they pull from into
   // f_tmp, then step n + 2 of the rim tiles into f_dst, on the aux stream
-  const bool f64 = L->storage == D2Q9_STORE_F64;
-  if (int r = f64 ? dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(L_rim_a, f_src, f_tmp, mom_src,
-                                                                mom_tmp, 0, a->stream, false)
-                  : dispatch_model<D2Q9_STORE_F32, PHASE_FUSED>(L_rim_a, f_src, f_tmp, mom_src,
-                                                                mom_tmp, 0, a->stream, false))
-    return r;
-  if (int r = f64 ? dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(L_rim_b, f_tmp, f_dst, mom_tmp,
-                                                                mom_dst, write_moments,
-                                                                a->stream, false)
-                  : dispatch_model<D2Q9_STORE_F32, PHASE_FUSED>(L_rim_b, f_tmp, f_dst, mom_tmp,
-                                                                mom_dst, write_moments,
-                                                                a->stream, false))
-    return r;
+  auto rim = [&](const d2q9_lattice* X, const void* a0, void* a1, const void* a2, void* a3,
+                 int wm) {
+    switch (L->storage) {
+      case D2Q9_STORE_F64: return dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(X, a0, a1, a2, a3, wm, a->stream, false);
+      case D2Q9_STORE_F32: return dispatch_model<D2Q9_STORE_F32, PHASE_FUSED>(X, a0, a1, a2, a3, wm, a->stream, false);
+      case D2Q9_STORE_F16DEV: return dispatch_model<D2Q9_STORE_F16DEV, PHASE_FUSED>(X, a0, a1, a2, a3, wm, a->stream, false);
+      default: return dispatch_model<D2Q9_STORE_BF16DEV, PHASE_FUSED>(X, a0, a1, a2, a3, wm, a->stream, false);
+    }
+  };
+  if (int r = rim(L_rim_a, f_src, f_tmp, mom_src, mom_tmp, 0)) return r;
+  if (int r = rim(L_rim_b, f_tmp, f_dst, mom_tmp, mom_dst, write_moments)) return r;
   if (n_deep > 0) {
     const dim3 grid((unsigned)n_deep), block(64, DEEP_WAVES);
     const int key = L->collision * 3 + L->forcing;
@@ -670,10 +675,14 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
     D2Q9_DEEP(SV, 5, D2Q9_MRT, D2Q9_GUO_SECOND_ORDER)              \
     default: return fail(D2Q9_E_ARG, "bad collision/forcing");     \
   }
-    if (f64) {
+    if (L->storage == D2Q9_STORE_F64) {
       D2Q9_DEEP_ALL(D2Q9_STORE_F64)
-    } else {
+    } else if (L->storage == D2Q9_STORE_F32) {
       D2Q9_DEEP_ALL(D2Q9_STORE_F32)
+    } else if (L->storage == D2Q9_STORE_F16DEV) {
+      D2Q9_DEEP_ALL(D2Q9_STORE_F16DEV)
+    } else {
+      D2Q9_DEEP_ALL(D2Q9_STORE_BF16DEV)
     }
 #undef D2Q9_DEEP_ALL
 #undef D2Q9_DEEP

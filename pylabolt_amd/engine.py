@@ -481,9 +481,8 @@ class D2Q9Engine:
     "bf16dev") and an optional per-cell ``force_field`` (padded
     ``[size, 2]``) that replaces ``rho * gravity``.  ``solid_tiles`` picks
     how tiles holding solids are swept (see ``CaseTables.set_tile``);
-    ``two_step`` (default: on for FP64 and FP32) advances pairs of steps
-    with one sweep over the deep interior (d2q9_fused_step2) when deep
-    tiles fit.
+    ``two_step`` (default: on) advances pairs of steps with one sweep over
+    the deep interior (d2q9_fused_step2) when deep tiles fit.
     """
 
     def __init__(self, nx, ny, solid_pad, density, velocity, elements,
@@ -594,11 +593,8 @@ class D2Q9Engine:
         self._upload_tiles()
         self.graph_steps = max(2, int(graph_steps) // 2 * 2)
         self._graphs = {}
-        # FP64 and FP32 advance pairs of steps with one sweep over the deep
-        # interior (the 16-bit forms are bound by arithmetic, which the
-        # grown tiles would add to)
-        self._two_step_req = (self.storage in (nt.STORE_F64, nt.STORE_F32)
-                              if two_step is None else bool(two_step))
+        # pairs of steps with one sweep over the deep interior
+        self._two_step_req = True if two_step is None else bool(two_step)
         self.two_step = False
         self._setup_two_step()
         self.time_step = 0
@@ -640,9 +636,6 @@ class D2Q9Engine:
         self.two_step = False
         if not self._two_step_req:
             return
-        if self.storage not in (nt.STORE_F64, nt.STORE_F32):
-            raise ValueError("the two-step sweep is built for FP64 and FP32 "
-                             "storage")
         d = self.tables.deep_split(int(self.lib.d2q9_deep_rows()))
         if d.n_deep == 0:
             return
@@ -658,11 +651,10 @@ class D2Q9Engine:
                          if d.wall_index is not None else None),
             }
             if not hasattr(self, "f_tmp"):
-                # ghost and solid cells of the scratch lattice hold the
-                # encoded zero (0.0 in FP64 and FP32 storage) like those of
-                # f[0] and f[1]
-                self.f_tmp = torch.zeros(self.pop_len, dtype=self.raw_dtype,
-                                         device=dev)
+                # initialize() copies f[0] into it: ghost and solid cells
+                # of the scratch lattice hold the encoded zero, like those
+                # of f[0] and f[1]
+                self.f_tmp = torch.empty_like(self.f[0])
                 self.mom_tmp = torch.zeros(3 * self.plane, dtype=self.ctype,
                                            device=dev)
         t = self._deep
@@ -740,6 +732,9 @@ class D2Q9Engine:
         nt.check(self.lib.d2q9_init_eq(self._Lp, nt.ptr(self.f[0]),
                                        nt.ptr(self.f[1]), nt.ptr(self.mom[0]),
                                        self._stream_ptr), "d2q9_init_eq")
+        if hasattr(self, "f_tmp"):
+            with torch.cuda.stream(self.stream):
+                self.f_tmp.copy_(self.f[0])
         # both buffers hold pop_new[0] == pop[0]
         self.i_pop = 1
         self.i_pop_new = 0
