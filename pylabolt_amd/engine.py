@@ -60,6 +60,9 @@ def geometry(nx, ny):
     return pitch, plane
 
 
+# default threshold of the two-step sweep: the last-level cache (256 MiB)
+TWO_STEP_MIN_BYTES = 256 << 20
+
 CPL = {nt.STORE_F64: 1, nt.STORE_F32: 2, nt.STORE_F16DEV: 2,
        nt.STORE_BF16DEV: 2}
 
@@ -481,8 +484,9 @@ class D2Q9Engine:
     "bf16dev") and an optional per-cell ``force_field`` (padded
     ``[size, 2]``) that replaces ``rho * gravity``.  ``solid_tiles`` picks
     how tiles holding solids are swept (see ``CaseTables.set_tile``);
-    ``two_step`` (default: on) advances pairs of steps with one sweep over
-    the deep interior (d2q9_fused_step2) when deep tiles fit.
+    ``two_step`` advances pairs of steps with one sweep over the deep
+    interior (d2q9_fused_step2) when deep tiles fit; by default when the
+    two population buffers exceed ``TWO_STEP_MIN_BYTES``.
     """
 
     def __init__(self, nx, ny, solid_pad, density, velocity, elements,
@@ -593,8 +597,14 @@ class D2Q9Engine:
         self._upload_tiles()
         self.graph_steps = max(2, int(graph_steps) // 2 * 2)
         self._graphs = {}
-        # pairs of steps with one sweep over the deep interior
-        self._two_step_req = True if two_step is None else bool(two_step)
+        # pairs of steps with one sweep over the deep interior, by default
+        # for lattices whose two population buffers exceed the 256 MiB
+        # last-level cache (smaller ones are served from it, and the extra
+        # launches of the two-step sweep cost more than the traffic saved)
+        if two_step is None:
+            two_step = (2 * self.f[0].numel() * self.f[0].element_size()
+                        > TWO_STEP_MIN_BYTES)
+        self._two_step_req = bool(two_step)
         self.two_step = False
         self._setup_two_step()
         self.time_step = 0

@@ -98,3 +98,49 @@ def test_two_step_off_without_deep_tiles():
     s = _solver(cases.lid_driven_cavity(20, 30), True)
     assert not s.engine.two_step
     s.advance(5)
+
+
+ABS_TOL = {"f32": 2e-6, "f16dev": 1e-4, "bf16dev": 1e-3}
+
+
+@pytest.mark.parametrize("storage", sorted(ABS_TOL))
+@pytest.mark.parametrize("name", ["cylinder", "cavity"])
+def test_two_step_reduced_precision_close_to_oracle(name, storage):
+    """The float forms through the two-step sweep against the FP64 oracle
+    after 20 steps, at the tolerances of the single-step precision tests
+    (16-bit storage rounds only the stored steps here, so it is at least as
+    close)."""
+    sim = _rough(CASES[name]())
+    case = oracle_case(sim)
+    O.initialize_pop(case)
+    s = _solver(sim, True, storage=storage)
+    assert s.engine.two_step
+    s.advance(20)
+    O.run(case, 20)
+    fl = case.fluid
+    err_pop = float(np.abs(s.engine.pop()[fl] - case.pop[fl]).max())
+    err_new = float(np.abs(s.engine.pop_new()[fl] - case.pop_new[fl]).max())
+    _, vel = s.engine.moments_host()
+    err_u = float(np.abs(vel[fl] - case.velocity[fl]).max())
+    print(f"[two-step] {name} {storage}: pop {err_pop:.2e} pop_new "
+          f"{err_new:.2e} u {err_u:.2e}", flush=True)
+    tol = ABS_TOL[storage]
+    assert err_pop <= tol and err_new <= tol and err_u <= 10 * tol
+
+
+@pytest.mark.parametrize("storage", ["f16dev", "bf16dev"])
+def test_two_step_16bit_reproducible_and_mode_independent(storage):
+    """16-bit two-step runs repeat bit for bit, whichever way the tiles
+    holding solids are swept."""
+    from pylabolt_amd.solvers.fluidLB import Solver
+    pops = []
+    for mode in ("sweep", "cells", "sweep"):
+        s = Solver(simulation=CASES["cylinder"](), storage=storage,
+                   graph_steps=8, verbose=False, two_step=True,
+                   solid_tiles=mode)
+        s.set_backend()
+        assert s.engine.two_step
+        s.advance(23)
+        pops.append(s.engine.pop())
+    assert np.array_equal(pops[0], pops[1])
+    assert np.array_equal(pops[0], pops[2])
