@@ -208,9 +208,10 @@ int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
                     const void* mom_src, void* mom_dst, int write_moments,
                     void* stream);
 
-/* Two fused steps (FP64 or FP32 storage): f_dst and mom_dst as after
- * d2q9_fused_step(f_src -> f_tmp) and d2q9_fused_step(f_tmp -> f_dst),
- * bitwise.  It has no counterpart in the reference: it replaces two of its
+/* Two fused steps: f_dst and mom_dst as after d2q9_fused_step(f_src ->
+ * f_tmp) and d2q9_fused_step(f_tmp -> f_dst), bitwise in FP64 and FP32
+ * storage; in 16-bit storage the deep tiles' intermediate step stays in
+ * FP32 (only the stored step is rounded).  It has no counterpart in the reference: it replaces two of its
  * time steps (solvers/fluidLB.py:206-253) by one sweep over the interior.
  * deep_tiles lists tiles of d2q9_deep_rows() rows x tile_j cells (tile t:
  * rows 1 + (t / ntj) * rows .., cells 1 + (t % ntj) * tile_j .., ntj =
