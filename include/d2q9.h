@@ -217,7 +217,8 @@ int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
  * rows 1 + (t / ntj) * rows .., cells 1 + (t % ntj) * tile_j .., ntj =
  * ceil(ny / tile_j)) whose cells and whose neighbours one cell deep are plain
  * fluid (flag D2Q9_CELL_FLUID only, no periodic edge); the caller
- * guarantees it.  Each is swept once for both steps, step n + 1 of the tile
+ * guarantees it (a tile whose grown region leaves the lattice is skipped,
+ * so a bad list cannot make an access out of bounds).  Each is swept once for both steps, step n + 1 of the tile
  * grown by one cell held in LDS.  L_rim_a and L_rim_b describe the same
  * lattice as L with tile and fix-up lists restricted to the rest (the rim):
  * L_rim_b covers every cell outside the deep tiles, L_rim_a those cells and

@@ -155,6 +155,9 @@ __global__ __launch_bounds__(64 * DEEP_WAVES) void k_deep2(
   const int t = deep[xcd_remap((int)blockIdx.x, n_deep)];
   const int tj = t % ntj;
   const int i0 = 1 + (t / ntj) * DEEP_ROWS, j0 = 1 + tj * B;
+  // the whole workgroup leaves (before its barrier) for a tile whose grown
+  // region is not inside the lattice: every access then stays in bounds
+  if (t < 0 || i0 - 1 < 1 || i0 + DEEP_ROWS > L.nx || j0 - 1 < 1 || j0 + B > L.ny) return;
   const Consts<T> K = load_consts<T>(L);
   if (w < DEEP_R1) {
     // grown row i0 - 1 + w, the block's B cells
@@ -626,7 +629,6 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
         X->forcing != L->forcing || !X->tiles)
       return fail(D2Q9_E_ARG, "rim descriptors must describe the same lattice, with tile lists");
   }
-
   if (n_deep < 0 || (n_deep > 0 && !deep_tiles)) return fail(D2Q9_E_ARG, "bad deep tile list");
   if (!f_src || !f_dst || !f_tmp || !mom_src || !mom_dst || !mom_tmp)
     return fail(D2Q9_E_ARG, "null buffer");
@@ -644,13 +646,18 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
     return r;
   // rim: step n + 1 of the rim tiles and the cells they pull from into
   // f_tmp, then step n + 2 of the rim tiles into f_dst, on the aux stream
-  auto rim = [&](const d2q9_lattice* X, const void* a0, void* a1, const void* a2, void* a3,
+  auto rim = [&](const d2q9_lattice* X, const void* fs, void* fd, const void* ms, void* md,
                  int wm) {
+    hipStream_t as = a->stream;
     switch (L->storage) {
-      case D2Q9_STORE_F64: return dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(X, a0, a1, a2, a3, wm, a->stream, false);
-      case D2Q9_STORE_F32: return dispatch_model<D2Q9_STORE_F32, PHASE_FUSED>(X, a0, a1, a2, a3, wm, a->stream, false);
-      case D2Q9_STORE_F16DEV: return dispatch_model<D2Q9_STORE_F16DEV, PHASE_FUSED>(X, a0, a1, a2, a3, wm, a->stream, false);
-      default: return dispatch_model<D2Q9_STORE_BF16DEV, PHASE_FUSED>(X, a0, a1, a2, a3, wm, a->stream, false);
+      case D2Q9_STORE_F64:
+        return dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(X, fs, fd, ms, md, wm, as, false);
+      case D2Q9_STORE_F32:
+        return dispatch_model<D2Q9_STORE_F32, PHASE_FUSED>(X, fs, fd, ms, md, wm, as, false);
+      case D2Q9_STORE_F16DEV:
+        return dispatch_model<D2Q9_STORE_F16DEV, PHASE_FUSED>(X, fs, fd, ms, md, wm, as, false);
+      default:
+        return dispatch_model<D2Q9_STORE_BF16DEV, PHASE_FUSED>(X, fs, fd, ms, md, wm, as, false);
     }
   };
   if (int r = rim(L_rim_a, f_src, f_tmp, mom_src, mom_tmp, 0)) return r;
