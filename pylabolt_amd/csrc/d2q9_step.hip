@@ -188,29 +188,36 @@ __global__ __launch_bounds__(64 * DEEP_WAVES) void k_deep2(
   if (w >= 1 && w <= DEEP_ROWS) {
     const int i = i0 - 1 + w, jl = j0 + lane * CPL;
     const int64_t c = cell_index(L, i, jl);
-    R out[9][CPL];
+    T g[CPL][9];
 #pragma unroll
     for (int q = 0; q < CPL; ++q) {
-      T g[9];
 #pragma unroll
-      for (int k = 0; k < 9; ++k) g[k] = s1[w - CX[k]][k][1 + lane * CPL + q - CY[k]];
+      for (int k = 0; k < 9; ++k) g[q][k] = s1[w - CX[k]][k][1 + lane * CPL + q - CY[k]];
       T rho, ux, uy;
-      deep_collide<S, MODEL, FORCING>(L, K, g, c + q, rho, ux, uy);
-      if constexpr (C::dev && sizeof(R) == 2) {
-        const uint32_t key = cell_key((uint32_t)(c + q));
-#pragma unroll
-        for (int k = 0; k < 9; ++k) out[k][q] = C::encs(g[k], k, key, K);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) out[k][q] = C::enc(g[k], k, K);
-      }
+      deep_collide<S, MODEL, FORCING>(L, K, g[q], c + q, rho, ux, uy);
       if (write_mom) {
         mdst[c + q] = rho;
         mdst[L.plane + c + q] = ux;
         mdst[2 * L.plane + c + q] = uy;
       }
     }
-    store_block<S>(L, fdst, i, tj, lane, out);
+    if constexpr (C::dev && sizeof(R) == 2 && CPL == 2) {
+      // the lane's two cells rounded and packed together, as in the bulk
+      // sweep
+      const uint32_t key0 = cell_key((uint32_t)c), key1 = cell_key((uint32_t)(c + 1));
+      const int64_t o = (((int64_t)i * L.nbj + tj + 1) * 9) * B + lane * CPL;
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        __builtin_nontemporal_store(C::encs2(g[0][k], g[1][k], k, key0, key1, K),
+                                    reinterpret_cast<uint32_t*>(fdst + o + (int64_t)k * B));
+    } else {
+      R out[9][CPL];
+#pragma unroll
+      for (int q = 0; q < CPL; ++q)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) out[k][q] = C::enc(g[q][k], k, K);
+      store_block<S>(L, fdst, i, tj, lane, out);
+    }
   }
 }
 
