@@ -160,10 +160,16 @@ __global__ __launch_bounds__(64 * DEEP_WAVES) void k_deep2(
   if (t < 0 || i0 - 1 < 1 || i0 + DEEP_ROWS > L.nx || j0 - 1 < 1 || j0 + B > L.ny) return;
   const Consts<T> K = load_consts<T>(L);
   if (w < DEEP_R1) {
-    // grown row i0 - 1 + w, the block's B cells
+    // grown row i0 - 1 + w, the block's B cells; the loads issue at raised
+    // wave priority, so that they reach memory ahead of other waves'
+    // arithmetic (measured: 0.7 % on the FP64 step)
     const int i = i0 - 1 + w, jl = j0 + lane * CPL;
     T g[CPL][9];
-    gather_block<S, PHASE_FUSED>(L, K, fsrc, i, tj, lane, jl, g);
+    __builtin_amdgcn_s_setprio(2);
+    RawBlock<S> rb;
+    gather_issue<S, PHASE_FUSED>(L, fsrc, i, tj, lane, jl, rb);
+    __builtin_amdgcn_s_setprio(0);
+    gather_finish<S, PHASE_FUSED>(K, rb, lane, g);
     const int64_t c = cell_index(L, i, jl);
 #pragma unroll
     for (int q = 0; q < CPL; ++q) {
