@@ -218,7 +218,12 @@ int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
  * ceil(ny / tile_j)) whose cells and whose neighbours one cell deep are plain
  * fluid (flag D2Q9_CELL_FLUID only, no periodic edge); the caller
  * guarantees it (a tile whose grown region leaves the lattice is skipped,
- * so a bad list cannot make an access out of bounds).  Each is swept once for both steps, step n + 1 of the tile
+ * so a bad list cannot make an access out of bounds).  deep_walls (null:
+ * none) gives per deep tile the straight wall its grown region may hold at
+ * j = 1 (bits 0-1) and at j = ny (bits 2-3): 1 zero-gradient, 2
+ * bounce-back, every cell of that column in the grown rows a wall cell of
+ * wall_masks sides 0/2 resp. 1/3; the ghost column beside it is then the
+ * only cell of the grown region outside the lattice.  Each is swept once for both steps, step n + 1 of the tile
  * grown by one cell held in LDS.  L_rim_a and L_rim_b describe the same
  * lattice as L with tile and fix-up lists restricted to the rest (the rim):
  * L_rim_b covers every cell outside the deep tiles, L_rim_a those cells and
@@ -228,7 +233,8 @@ int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
  * population buffers and all three moment buffers must differ. */
 int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
                      const d2q9_lattice* L_rim_b, const int32_t* deep_tiles,
-                     int32_t n_deep, const void* f_src, void* f_dst, void* f_tmp,
+                     const uint8_t* deep_walls, int32_t n_deep, const void* f_src,
+                     void* f_dst, void* f_tmp,
                      const void* mom_src, void* mom_dst, void* mom_tmp,
                      int write_moments, void* stream);
 

@@ -107,9 +107,9 @@ SIGNATURES = {
     "d2q9_collide": (ctypes.c_int, [_LP, _P, _P, _P, _P, ctypes.c_int, _P]),
     "d2q9_stream_bc": (ctypes.c_int, [_LP, _P, _P, _P, _P]),
     "d2q9_fused_step": (ctypes.c_int, [_LP, _P, _P, _P, _P, ctypes.c_int, _P]),
-    "d2q9_fused_step2": (ctypes.c_int, [_LP, _LP, _LP, _P, ctypes.c_int32,
-                                        _P, _P, _P, _P, _P, _P, ctypes.c_int,
-                                        _P]),
+    "d2q9_fused_step2": (ctypes.c_int, [_LP, _LP, _LP, _P, _P,
+                                        ctypes.c_int32, _P, _P, _P, _P, _P,
+                                        _P, ctypes.c_int, _P]),
     "d2q9_deep_rows": (ctypes.c_int, []),
     "d2q9_moments": (ctypes.c_int, [_LP, _P, _P, _P]),
     "d2q9_reduce_work_len": (ctypes.c_int64, [_LP]),

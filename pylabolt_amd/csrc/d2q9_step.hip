@@ -140,11 +140,31 @@ __device__ __forceinline__ void deep_collide(const d2q9_lattice& L,
   collide<T, MODEL, FORCING, Codec<S>::dev>(g, K, drho, rho, ux, uy, fx, fy);
 }
 
+// Wall kinds of a deep tile at the low-y (j = 1) and high-y (j = ny) walls:
+// 0 none, 1 zero-gradient, 2 bounce-back.  Each wall cell's unknown
+// directions (2,5,6 at j = 1; 4,7,8 at j = ny) take, for zero-gradient, the
+// value its inner neighbour pulled (which lies at the wall cell's own
+// column) and, for bounce-back, the cell's own post-collision value of the
+// opposite direction: the rules of the bulk sweep's wall rows.
+template <int S>
+__device__ __forceinline__ void deep_wall_global(const d2q9_lattice& L,
+                                                 const Consts<typename Codec<S>::T>& K,
+                                                 const typename Codec<S>::raw* __restrict__ fsrc,
+                                                 typename Codec<S>::T (&g)[9], int kind, int i,
+                                                 int j, int cy) {
+#pragma unroll
+  for (int d = 1; d < 9; ++d) {
+    if (CY[d] != cy) continue;
+    g[d] = kind == 1 ? ld<S>(L, K, fsrc, i - CX[d], j, d) : ld<S>(L, K, fsrc, i, j, INV[d]);
+  }
+}
+
 template <int S, int MODEL, int FORCING>
 __global__ __launch_bounds__(64 * DEEP_WAVES) void k_deep2(
-    const d2q9_lattice L, const int32_t* __restrict__ deep, int n_deep, int ntj,
-    const typename Codec<S>::raw* __restrict__ fsrc, typename Codec<S>::raw* __restrict__ fdst,
-    typename Codec<S>::T* __restrict__ mdst, int write_mom) {
+    const d2q9_lattice L, const int32_t* __restrict__ deep, const uint8_t* __restrict__ walls,
+    int n_deep, int ntj, const typename Codec<S>::raw* __restrict__ fsrc,
+    typename Codec<S>::raw* __restrict__ fdst, typename Codec<S>::T* __restrict__ mdst,
+    int write_mom) {
   using C = Codec<S>;
   using T = typename C::T;
   using R = typename C::raw;
@@ -152,12 +172,18 @@ __global__ __launch_bounds__(64 * DEEP_WAVES) void k_deep2(
   constexpr int B = 64 * CPL;
   __shared__ T s1[DEEP_R1][9][B + 2];
   const int lane = (int)threadIdx.x, w = (int)threadIdx.y;
-  const int t = deep[xcd_remap((int)blockIdx.x, n_deep)];
+  const int p = xcd_remap((int)blockIdx.x, n_deep);
+  const int t = deep[p];
+  const int wf = walls ? (int)walls[p] : 0;
+  const int lo = wf & 3, hi = (wf >> 2) & 3;
   const int tj = t % ntj;
   const int i0 = 1 + (t / ntj) * DEEP_ROWS, j0 = 1 + tj * B;
   // the whole workgroup leaves (before its barrier) for a tile whose grown
-  // region is not inside the lattice: every access then stays in bounds
-  if (t < 0 || i0 - 1 < 1 || i0 + DEEP_ROWS > L.nx || j0 - 1 < 1 || j0 + B > L.ny) return;
+  // region is not inside the lattice, but for the ghost column beside a
+  // wall: every access then stays in bounds
+  if (t < 0 || i0 - 1 < 1 || i0 + DEEP_ROWS > L.nx || (j0 - 1 < 1 && !(lo && j0 == 1)) ||
+      (j0 + B > L.ny && !(hi && j0 + B - 1 == L.ny)))
+    return;
   const Consts<T> K = load_consts<T>(L);
   if (w < DEEP_R1) {
     // grown row i0 - 1 + w, the block's B cells; the loads issue at raised
@@ -170,6 +196,8 @@ __global__ __launch_bounds__(64 * DEEP_WAVES) void k_deep2(
     gather_issue<S, PHASE_FUSED>(L, fsrc, i, tj, lane, jl, rb);
     __builtin_amdgcn_s_setprio(0);
     gather_finish<S, PHASE_FUSED>(K, rb, lane, g);
+    if (lo && lane == 0) deep_wall_global<S>(L, K, fsrc, g[0], lo, i, 1, 1);
+    if (hi && lane == 63) deep_wall_global<S>(L, K, fsrc, g[CPL - 1], hi, i, L.ny, -1);
     const int64_t c = cell_index(L, i, jl);
 #pragma unroll
     for (int q = 0; q < CPL; ++q) {
@@ -178,8 +206,8 @@ __global__ __launch_bounds__(64 * DEEP_WAVES) void k_deep2(
 #pragma unroll
       for (int k = 0; k < 9; ++k) s1[w][k][1 + lane * CPL + q] = g[q][k];
     }
-  } else if (lane < 2 * DEEP_R1) {
-    // grown columns j0 - 1 and j0 + B
+  } else if (lane < 2 * DEEP_R1 && !((lane & 1) ? hi : lo)) {
+    // grown columns j0 - 1 and j0 + B (not the ghost column beside a wall)
     const int r = lane >> 1, cl = (lane & 1) ? B + 1 : 0;
     const int i = i0 - 1 + r, j = j0 - 1 + cl;
     T g[9];
@@ -197,8 +225,19 @@ __global__ __launch_bounds__(64 * DEEP_WAVES) void k_deep2(
     T g[CPL][9];
 #pragma unroll
     for (int q = 0; q < CPL; ++q) {
+      const int col = 1 + lane * CPL + q;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) g[q][k] = s1[w - CX[k]][k][1 + lane * CPL + q - CY[k]];
+      for (int k = 0; k < 9; ++k) g[q][k] = s1[w - CX[k]][k][col - CY[k]];
+      const int wk = (lo && lane == 0 && q == 0) ? lo : 0;
+      const int wkh = (hi && lane == 63 && q == CPL - 1) ? hi : 0;
+      if (wk | wkh) {
+        const int kind = wk ? wk : wkh, cyw = wk ? 1 : -1;
+#pragma unroll
+        for (int d = 1; d < 9; ++d) {
+          if (CY[d] != cyw) continue;
+          g[q][d] = kind == 1 ? s1[w - CX[d]][d][col] : s1[w][INV[d]][col];
+        }
+      }
       T rho, ux, uy;
       deep_collide<S, MODEL, FORCING>(L, K, g[q], c + q, rho, ux, uy);
       if (write_mom) {
@@ -630,7 +669,8 @@ int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
 
 int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
                      const d2q9_lattice* L_rim_b, const int32_t* deep_tiles,
-                     int32_t n_deep, const void* f_src, void* f_dst, void* f_tmp,
+                     const uint8_t* deep_walls, int32_t n_deep, const void* f_src,
+                     void* f_dst, void* f_tmp,
                      const void* mom_src, void* mom_dst, void* mom_tmp,
                      int write_moments, void* stream) {
   for (const d2q9_lattice* X : {L, L_rim_a, L_rim_b})
@@ -680,7 +720,8 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
     const int key = L->collision * 3 + L->forcing;
 #define D2Q9_DEEP(SV, KEY, M, F)                                                         \
   case KEY:                                                                              \
-    hipLaunchKernelGGL((k_deep2<SV, M, F>), grid, block, 0, s, *L, deep_tiles, n_deep, ntj, \
+    hipLaunchKernelGGL((k_deep2<SV, M, F>), grid, block, 0, s, *L, deep_tiles, deep_walls,   \
+                       n_deep, ntj,                                                      \
                        static_cast<const Codec<SV>::raw*>(f_src),                        \
                        static_cast<Codec<SV>::raw*>(f_dst),                              \
                        static_cast<Codec<SV>::T*>(mom_dst), write_moments);              \

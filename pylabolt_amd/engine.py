@@ -318,16 +318,22 @@ class CaseTables:
         tables around them.
 
         A deep tile is ``rows`` x one block (``tile_j`` cells) of interior
-        cells (block-aligned, rows a multiple of the tile rows) whose cells and neighbours one cell deep
-        are all plain fluid: no boundary rule, solid neighbour, stored
-        moment or periodic edge, and inside the lattice.  The rim is every
+        cells (block-aligned, rows a multiple of the tile rows) whose cells
+        and neighbours one cell deep are all plain fluid -- no boundary
+        rule, solid neighbour, stored moment or periodic edge, and inside
+        the lattice -- except that the first or last column of the lattice
+        may be a straight zero-gradient or bounce-back wall swept in the
+        tile (wall rows of sides 0-3, one kind along the tile's rows; then
+        the ghost column beside it is the one cell of the grown region
+        outside the lattice; ``deep_walls`` gives the kinds).  The rim is every
         other cell, swept by whole tiles of the current tile shape (a tile
         is either inside one deep tile or in the rim).  Rim B lists the rim
         tiles with their fix-up cells; rim A lists the same tiles and adds
         to the fix-up list every deep cell a rim cell pulls from, so that
         its first step leaves in the scratch lattice all that the second
-        reads.  Returns a namespace (deep_ids, n_deep, tile_list, n_bulk,
-        n_general, bulk_masks, wall_index, fix_a, fix_b)."""
+        reads.  Returns a namespace (deep_ids, deep_walls, n_deep,
+        tile_list, n_bulk, n_general, bulk_masks, wall_index, fix_a,
+        fix_b)."""
         B = self.tile_j
         if rows % self.tile_i:
             raise ValueError("deep tile rows must be a multiple of the tile "
@@ -349,13 +355,44 @@ class CaseTables:
         # rows, cells tj*B - 1 .. tj*B + B
         r0, r1 = ti * rows - 1, ti * rows + rows
         c0, c1 = tj * B - 1, tj * B + B
-        ok = (r0 >= 0) & (r1 <= nx - 1) & (c0 >= 0) & (c1 <= ny - 1)
+        # a tile at the low (high) wall: its grown region reaches the ghost
+        # column, and the wall column is checked apart from the plain cells
+        at_lo = (c0 == -1) & np.ones_like(r0, dtype=bool)
+        at_hi = (c1 == ny) & np.ones_like(r0, dtype=bool)
+        ca = np.where(at_lo, 1, c0)
+        cb = np.where(at_hi, ny - 2, c1)
+        ok = ((r0 >= 0) & (r1 <= nx - 1) & (ca >= 0) & (cb <= ny - 1) &
+              (ca <= cb))
         r0c, r1c = np.clip(r0, 0, nx - 1), np.clip(r1, 0, nx - 1)
-        c0c, c1c = np.clip(c0, 0, ny - 1), np.clip(c1, 0, ny - 1)
-        bad = (sat[r1c + 1, c1c + 1] - sat[r0c, c1c + 1] - sat[r1c + 1, c0c]
-               + sat[r0c, c0c])
+        cac, cbc = np.clip(ca, 0, ny - 1), np.clip(cb, 0, ny - 1)
+        bad = (sat[r1c + 1, cbc + 1] - sat[r0c, cbc + 1] - sat[r1c + 1, cac]
+               + sat[r0c, cac])
         ok &= bad == 0
+        # wall kinds per row: 1 zero-gradient, 2 bounce-back, 0 neither (a
+        # wall cell the bulk sweep does inline, sides 0/2 at j = 1, 1/3 at
+        # j = ny)
+        side = self.wall_side_pad[1:-1, 1:-1]
+        inl = self.wall_inline
+        lo_row = np.where(inl[:, 0] & (side[:, 0] == 0), 1,
+                          np.where(inl[:, 0] & (side[:, 0] == 2), 2, 0))
+        hi_row = np.where(inl[:, -1] & (side[:, -1] == 1), 1,
+                          np.where(inl[:, -1] & (side[:, -1] == 3), 2, 0))
+        n_rows = r1c - r0c + 1
+
+        def kind_over_rows(per_row):
+            kind = np.zeros_like(r0)
+            for k in (1, 2):
+                cum = np.concatenate([[0], np.cumsum(per_row == k)])
+                full = (cum[r1c + 1] - cum[r0c]) == n_rows
+                kind = np.where(full, k, kind)
+            return kind
+
+        lo_kind = np.where(at_lo, kind_over_rows(lo_row), 0)
+        hi_kind = np.where(at_hi, kind_over_rows(hi_row), 0)
+        ok &= ~at_lo | (lo_kind > 0)
+        ok &= ~at_hi | (hi_kind > 0)
         deep_ids = (ti * ntj + tj)[ok].astype(np.int32)
+        deep_walls = (lo_kind | (hi_kind << 2))[ok].astype(np.uint8)
         covered = np.repeat(np.repeat(ok, rows, axis=0), B,
                             axis=1)[:nx, :ny]
         # rim tiles of the current tile shape
@@ -420,6 +457,8 @@ class CaseTables:
         fb, nfb = as_list(fix_b)
         return SimpleNamespace(
             deep_ids=deep_ids if deep_ids.size else np.zeros(1, np.int32),
+            deep_walls=(deep_walls if deep_walls.size
+                        else np.zeros(1, np.uint8)),
             n_deep=int(deep_ids.size), rows=int(rows), covered=covered,
             tile_list=tile_list, n_bulk=int(bulk_pos.size),
             n_general=int(gen.size), bulk_masks=bulk_masks,
@@ -653,6 +692,7 @@ class D2Q9Engine:
         with torch.cuda.stream(self.stream):
             self._deep = {
                 "deep": torch.from_numpy(d.deep_ids).to(dev),
+                "walls": torch.from_numpy(d.deep_walls).to(dev),
                 "tiles": torch.from_numpy(d.tile_list).to(dev),
                 "masks": torch.from_numpy(d.bulk_masks.view(np.int64)).to(dev),
                 "fix_a": torch.from_numpy(d.fix_a).to(dev),
@@ -781,7 +821,8 @@ class D2Q9Engine:
     def _fused2(self, src, dst, write_mom):
         nt.check(self.lib.d2q9_fused_step2(
             self._Lp, self._rim_ap, self._rim_bp,
-            nt.ptr(self._deep["deep"]), self.n_deep,
+            nt.ptr(self._deep["deep"]), nt.ptr(self._deep["walls"]),
+            self.n_deep,
             nt.ptr(self.f[src]), nt.ptr(self.f[dst]), nt.ptr(self.f_tmp),
             nt.ptr(self.mom[self.i_mom]), nt.ptr(self.mom[1 - self.i_mom]),
             nt.ptr(self.mom_tmp), int(write_mom), self._stream_ptr),
@@ -876,7 +917,8 @@ class D2Q9Engine:
             if self.two_step:
                 rc = self.lib.d2q9_fused_step2(
                     self._Lp, self._rim_ap, self._rim_bp,
-                    nt.ptr(self._deep["deep"]), self.n_deep,
+                    nt.ptr(self._deep["deep"]), nt.ptr(self._deep["walls"]),
+                    self.n_deep,
                     nt.ptr(self.f[src]), nt.ptr(self.f[1 - src]),
                     nt.ptr(self.f_tmp), nt.ptr(self.mom[im]),
                     nt.ptr(self.mom[1 - im]), nt.ptr(self.mom_tmp), 0,

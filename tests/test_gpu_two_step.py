@@ -25,8 +25,8 @@ def _rough(sim):
     return sim
 
 
-def _forced_channel():
-    sim = cases.poiseuille_channel(80, 300, model="MRT",
+def _forced_channel(ny=300):
+    sim = cases.poiseuille_channel(80, ny, model="MRT",
                                    forcing="guo_second_order")
     sim.forcing_dict["gravity"] = [2e-5, 3e-6]
     return sim
@@ -38,6 +38,11 @@ CASES = {
     "channel_mrt_guo2": _forced_channel,
     "cylinder_mrt": lambda: cases.cylinder(90, 260, diameter=16, xc=30,
                                            model="MRT"),
+    # ny a multiple of both block widths: deep tiles reach both walls
+    # (zero-gradient for the cylinder, bounce-back for the channel)
+    "cylinder_walls_both": lambda: cases.cylinder(100, 256, diameter=16,
+                                                  xc=50),
+    "channel_walls_both": lambda: _forced_channel(256),
 }
 
 
@@ -57,6 +62,9 @@ def test_two_step_matches_oracle(name):
     s = _solver(sim, True)
     eng = s.engine
     assert eng.two_step and eng.n_deep > 0
+    if name.endswith("walls_both"):
+        walls = eng._deep["walls"].cpu().numpy()[:eng.n_deep]
+        assert (walls & 3).any() and (walls >> 2).any()
     print(f"[two-step] {name}: {eng.n_deep} deep tiles, "
           f"{eng.deep_fraction:.0%} of the cells", flush=True)
     done = 0
@@ -76,7 +84,9 @@ def test_two_step_matches_oracle(name):
 
 
 @pytest.mark.parametrize("storage", ["f64", "f32"])
-@pytest.mark.parametrize("name", ["cylinder", "cavity", "channel_mrt_guo2"])
+@pytest.mark.parametrize("name", ["cylinder", "cavity", "channel_mrt_guo2",
+                                  "cylinder_walls_both",
+                                  "channel_walls_both"])
 def test_two_step_bitwise_equals_single_steps(name, storage):
     one = _solver(CASES[name](), False, storage=storage)
     two = _solver(CASES[name](), True, storage=storage)

@@ -179,16 +179,20 @@ def test_solid_tiles_as_fix_cells(B):
 
 DEEP_CASES = {
     "cylinder": lambda: cases.cylinder(120, 300, diameter=20, xc=40),
+    "cylinder_walls_both": lambda: cases.cylinder(100, 256, diameter=16,
+                                                  xc=50),
     "cavity": lambda: cases.lid_driven_cavity(100, 200),
-    "channel": lambda: cases.poiseuille_channel(80, 250, model="BGK"),
+    "channel": lambda: cases.poiseuille_channel(80, 256, model="BGK"),
 }
 
 
 @pytest.mark.parametrize("name", sorted(DEEP_CASES))
 def test_deep_split_covers_and_feeds_the_rim(name):
     """Deep tiles of the two-step sweep: block-aligned, disjoint, plain fluid
-    one cell around; every other cell lies in a listed rim tile; the first
-    rim step (list A) computes every cell the second (list B) pulls from."""
+    one cell around but for a flagged straight wall column (zero-gradient
+    or bounce-back wall cells swept inline); every other cell lies in a
+    listed rim tile; the first rim step (list A) computes every cell the
+    second (list B) pulls from."""
     from pylabolt_amd import _native as nt
     st = State(DEEP_CASES[name]())
     nx, ny = (int(v) for v in st.mesh.grid_global_shape)
@@ -207,11 +211,27 @@ def test_deep_split_covers_and_feeds_the_rim(name):
         plain[:, [0, -1]] = False
     ntj = -(-ny // 64)
     cov = np.zeros((nx, ny), dtype=int)
-    for tid in d.deep_ids[:d.n_deep]:
+    side = t.wall_side_pad[1:-1, 1:-1]
+    for tid, wf in zip(d.deep_ids[:d.n_deep], d.deep_walls[:d.n_deep]):
         ti, tj = divmod(int(tid), ntj)
+        lo, hi = int(wf) & 3, (int(wf) >> 2) & 3
         a, b = ti * rows, tj * 64
-        assert a + rows <= nx - 1 and a >= 1 and b >= 1 and b + 64 <= ny - 1
-        assert plain[a - 1:a + rows + 1, b - 1:b + 65].all()
+        assert a + rows <= nx - 1 and a >= 1
+        c0 = b - 1 if not lo else b + 1      # the wall column apart
+        c1 = b + 64 if not hi else b + 62
+        if lo:
+            assert b == 0
+            # the wall column: inline wall cells of the flagged kind
+            want = 0 if lo == 1 else 2
+            assert t.wall_inline[a - 1:a + rows + 1, 0].all()
+            assert (side[a - 1:a + rows + 1, 0] == want).all()
+        if hi:
+            assert b + 64 == ny
+            want = 1 if hi == 1 else 3
+            assert t.wall_inline[a - 1:a + rows + 1, ny - 1].all()
+            assert (side[a - 1:a + rows + 1, ny - 1] == want).all()
+        assert c0 >= 0 and c1 <= ny - 1
+        assert plain[a - 1:a + rows + 1, c0:c1 + 1].all()
         cov[a:a + rows, b:b + 64] += 1
     assert cov.max() == 1
     assert np.array_equal(cov == 1, d.covered)
