@@ -695,26 +695,6 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
   Aux* a = nullptr;
   if (int r = aux_for_current_device(&a)) return r;
   if (int r = check_hip(hipEventRecord(a->fork, s), "hipEventRecord fork")) return r;
-  if (int r = check_hip(hipStreamWaitEvent(a->stream, a->fork, 0), "hipStreamWaitEvent fork"))
-    return r;
-  // rim: step n + 1 of the rim tiles and the cells they pull from into
-  // f_tmp, then step n + 2 of the rim tiles into f_dst, on the aux stream
-  auto rim = [&](const d2q9_lattice* X, const void* fs, void* fd, const void* ms, void* md,
-                 int wm) {
-    hipStream_t as = a->stream;
-    switch (L->storage) {
-      case D2Q9_STORE_F64:
-        return dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(X, fs, fd, ms, md, wm, as, false);
-      case D2Q9_STORE_F32:
-        return dispatch_model<D2Q9_STORE_F32, PHASE_FUSED>(X, fs, fd, ms, md, wm, as, false);
-      case D2Q9_STORE_F16DEV:
-        return dispatch_model<D2Q9_STORE_F16DEV, PHASE_FUSED>(X, fs, fd, ms, md, wm, as, false);
-      default:
-        return dispatch_model<D2Q9_STORE_BF16DEV, PHASE_FUSED>(X, fs, fd, ms, md, wm, as, false);
-    }
-  };
-  if (int r = rim(L_rim_a, f_src, f_tmp, mom_src, mom_tmp, 0)) return r;
-  if (int r = rim(L_rim_b, f_tmp, f_dst, mom_tmp, mom_dst, write_moments)) return r;
   if (n_deep > 0) {
     const dim3 grid((unsigned)n_deep), block(64, DEEP_WAVES);
     const int key = L->collision * 3 + L->forcing;
@@ -749,6 +729,28 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
 #undef D2Q9_DEEP
     if (int r = check_hip(hipGetLastError(), "k_deep2 launch")) return r;
   }
+  if (int r = check_hip(hipStreamWaitEvent(a->stream, a->fork, 0), "hipStreamWaitEvent fork"))
+    return r;
+  // rim: step n + 1 of the rim tiles and the cells they pull from into
+  // f_tmp, then step n + 2 of the rim tiles into f_dst, on the aux stream
+  // (enqueued after the deep tiles, whose workgroups then reach the CUs
+  // first)
+  auto rim = [&](const d2q9_lattice* X, const void* fs, void* fd, const void* ms, void* md,
+                 int wm) {
+    hipStream_t as = a->stream;
+    switch (L->storage) {
+      case D2Q9_STORE_F64:
+        return dispatch_model<D2Q9_STORE_F64, PHASE_FUSED>(X, fs, fd, ms, md, wm, as, false);
+      case D2Q9_STORE_F32:
+        return dispatch_model<D2Q9_STORE_F32, PHASE_FUSED>(X, fs, fd, ms, md, wm, as, false);
+      case D2Q9_STORE_F16DEV:
+        return dispatch_model<D2Q9_STORE_F16DEV, PHASE_FUSED>(X, fs, fd, ms, md, wm, as, false);
+      default:
+        return dispatch_model<D2Q9_STORE_BF16DEV, PHASE_FUSED>(X, fs, fd, ms, md, wm, as, false);
+    }
+  };
+  if (int r = rim(L_rim_a, f_src, f_tmp, mom_src, mom_tmp, 0)) return r;
+  if (int r = rim(L_rim_b, f_tmp, f_dst, mom_tmp, mom_dst, write_moments)) return r;
   if (int r = check_hip(hipEventRecord(a->join, a->stream), "hipEventRecord join")) return r;
   return check_hip(hipStreamWaitEvent(s, a->join, 0), "hipStreamWaitEvent join");
 }
