@@ -105,7 +105,11 @@ __global__ __launch_bounds__(256) void k_step_all(
 
 // Two fused steps of a deep tile: DEEP_ROWS rows x one block (B = 64 * CPL
 // cells) whose neighbourhood two cells deep holds only plain fluid cells
-// (the host checks it).  Stage one computes step n + 1 (pull from the
+// (the host checks it), but for a straight zero-gradient or bounce-back
+// wall in the lattice's first or last column (deep_wall_global and the
+// wall case of stage two apply its rule; the ghost column beside it is not
+// computed, since only the wall's replaced directions would read it).
+// Stage one computes step n + 1 (pull from the
 // lattice, moments, collision) on the tile grown by one cell on every side
 // into LDS; stage two pulls step n + 1 from LDS and stores step n + 2.  Per
 // cell and two steps the lattice is read about 1.2 times (the grown rows
