@@ -154,3 +154,35 @@ def test_two_step_16bit_reproducible_and_mode_independent(storage):
         pops.append(s.engine.pop())
     assert np.array_equal(pops[0], pops[1])
     assert np.array_equal(pops[0], pops[2])
+
+
+def test_two_step_force_field_bitwise():
+    """A per-cell force field (the adjoint's forcing) through the two-step
+    sweep, FP64, Guo forcing, bounce-back walls reached by deep tiles:
+    bitwise equal to single steps."""
+    from pylabolt_amd.base.collision_operator import CollisionOperator
+    from pylabolt_amd.base.state import State
+    from pylabolt_amd.engine import D2Q9Engine, elements_from_boundary
+    from pylabolt_amd.solvers.fluidLB import FluidLB
+    sim, force, _, _ = cases.adjoint_channel(nx=96, ny=256)
+    st = State(sim)
+    co = CollisionOperator(sim, FluidLB(), st, verbose=False)
+    nx, ny = 96, 256
+    ff = np.zeros((nx + 2, ny + 2, 2))
+    ff[1:-1, 1:-1] = force * 50.0
+    lat = st.lattice
+    pops = []
+    for two in (False, True):
+        e = D2Q9Engine(nx, ny, st.fields.solid, st.fields.density,
+                       st.fields.velocity, elements_from_boundary(st.boundary),
+                       st.boundary.x_periodic, st.boundary.y_periodic, "BGK",
+                       co.forcing_fluid, co.omega_fluid, (0.0, 0.0),
+                       lat.inv_cs_2, lat.inv_cs_4, lat.weights,
+                       st.control.float_min, storage="f64",
+                       force_field=ff.reshape(-1, 2), graph_steps=8,
+                       two_step=two)
+        assert e.two_step == two
+        e.advance(31)
+        pops.append((e.pop(), e.moments_host()[1]))
+    assert np.array_equal(pops[0][0], pops[1][0])
+    assert np.array_equal(pops[0][1], pops[1][1])
