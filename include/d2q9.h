@@ -104,8 +104,9 @@ typedef struct d2q9_bc_node {
  *
  * Populations (AoSoA): direction k of cell (i, j) is element
  *   ((i * nbj + jj / B) * 9 + k) * B + jj % B,  jj = j + B - 1,
- * with B = 64 * cpl cells per block; cpl = 1 (FP64), 2 (FP32), 4 (16-bit)
- * so that one lane of a wave moves 8 bytes per direction.  A buffer holds
+ * with B = 64 * cpl cells per block; cpl = 1 (FP64), 2 (FP32 and 16-bit),
+ * so that one lane of a wave moves 8 bytes (4 bytes in 16-bit storage) per
+ * direction.  A buffer holds
  * pop_len = (nx + 2) * nbj * 9 * B elements, nbj >= (ny + B) / B + 1.
  *
  * Per-cell side arrays (flags, bc_slot, moments [rho|ux|uy], force field
