@@ -163,8 +163,15 @@ __device__ __forceinline__ void deep_wall_global(const d2q9_lattice& L,
   }
 }
 
+// Two workgroups (30 waves) per CU need 8 waves per SIMD, i.e. at most 64
+// VGPRs.  The float forms are held to that (MRT with Guo forcing: 67-84
+// VGPRs unbounded; bounded, at most 16 bytes per lane spill, and the FP32
+// 4096x1024 MRT channel measured 41.4 us per step against 51.0).  FP64
+// keeps the default for a 960-thread block (4 waves per SIMD): its MRT with
+// Guo forcing needs 94 VGPRs and, held to 64, spilled 196 bytes per lane
+// and ran 3x slower.
 template <int S, int MODEL, int FORCING>
-__global__ __launch_bounds__(64 * DEEP_WAVES) void k_deep2(
+__global__ __launch_bounds__(64 * DEEP_WAVES, Codec<S>::dev ? 8 : 4) void k_deep2(
     const d2q9_lattice L, const int32_t* __restrict__ deep, const uint8_t* __restrict__ walls,
     int n_deep, int ntj, const typename Codec<S>::raw* __restrict__ fsrc,
     typename Codec<S>::raw* __restrict__ fdst, typename Codec<S>::T* __restrict__ mdst,

@@ -114,7 +114,7 @@ ABS_TOL = {"f32": 2e-6, "f16dev": 1e-4, "bf16dev": 1e-3}
 
 
 @pytest.mark.parametrize("storage", sorted(ABS_TOL))
-@pytest.mark.parametrize("name", ["cylinder", "cavity"])
+@pytest.mark.parametrize("name", ["cylinder", "cavity", "channel_mrt_guo2"])
 def test_two_step_reduced_precision_close_to_oracle(name, storage):
     """The float forms through the two-step sweep against the FP64 oracle
     after 20 steps, at the tolerances of the single-step precision tests
