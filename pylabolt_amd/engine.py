@@ -60,8 +60,11 @@ def geometry(nx, ny):
     return pitch, plane
 
 
-# default threshold of the two-step sweep: the last-level cache (256 MiB)
-TWO_STEP_MIN_BYTES = 256 << 20
+# default threshold of the two-step sweep, on both population buffers
+# (measured on MRT channels: even from the 256 MiB last-level cache, 90 MiB
+# and more ran 6-19 % faster with it; 54 MiB broke even; 18-36 MiB ran up to
+# 1.8x slower, the rim's extra launches costing more than the traffic saved)
+TWO_STEP_MIN_BYTES = 64 << 20
 
 CPL = {nt.STORE_F64: 1, nt.STORE_F32: 2, nt.STORE_F16DEV: 2,
        nt.STORE_BF16DEV: 2}
@@ -637,9 +640,8 @@ class D2Q9Engine:
         self.graph_steps = max(2, int(graph_steps) // 2 * 2)
         self._graphs = {}
         # pairs of steps with one sweep over the deep interior, by default
-        # for lattices whose two population buffers exceed the 256 MiB
-        # last-level cache (smaller ones are served from it, and the extra
-        # launches of the two-step sweep cost more than the traffic saved)
+        # for lattices whose two population buffers exceed
+        # TWO_STEP_MIN_BYTES
         if two_step is None:
             two_step = (2 * self.f[0].numel() * self.f[0].element_size()
                         > TWO_STEP_MIN_BYTES)

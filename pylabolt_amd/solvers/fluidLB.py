@@ -66,7 +66,7 @@ class Solver:
     how tiles holding solid cells are swept (``solid_tiles``: "sweep",
     "cells" or "auto"; None picks per storage form) and whether pairs of
     steps run as one sweep over the deep interior (``two_step``; None: on
-    for lattices larger than the last-level cache).  Every choice gives bitwise the same results.
+    when the two population buffers exceed 64 MiB).  Every choice gives bitwise the same results.
     """
 
     def __init__(self, comm=None, backend="gpu", n_threads=None,
