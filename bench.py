@@ -67,6 +67,13 @@ def make_workload(name, nx, ny, steps):
     raise ValueError(name)
 
 
+def model_name(sim):
+    """Collision model and forcing of a case dict, e.g. "MRT+guo_second_order"."""
+    fl = sim.collision_dict["fluid"]
+    forcing = fl.get("forcing_model")
+    return fl["model"] + (f"+{forcing}" if forcing else "")
+
+
 def time_engine(solver, steps, warmup, use_graph=True):
     import torch
     eng = solver.engine
@@ -426,7 +433,8 @@ def run(real_stdout):
         "vs_baseline": None,
         "dtype": DTYPES[args.storage],
         "data": "synthetic (seeded case file, fluid at rest, rho=1)",
-        "config": {"workload": f"{args.workload} D2Q9 BGK {nx}x{ny}",
+        "config": {"workload": f"{args.workload} D2Q9 {model_name(sim)} "
+                               f"{nx}x{ny}",
                    "storage": args.storage, "cells": cells,
                    "tile": f"{tile[0]}x{tile[1]}", "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS,
