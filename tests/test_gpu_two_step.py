@@ -186,3 +186,21 @@ def test_two_step_force_field_bitwise():
         pops.append((e.pop(), e.moments_host()[1]))
     assert np.array_equal(pops[0][0], pops[1][0])
     assert np.array_equal(pops[0][1], pops[1][1])
+
+
+def test_two_step_default_threshold():
+    """By default the two-step sweep runs when the two population buffers
+    exceed TWO_STEP_MIN_BYTES: not on the 1024x256 FP64 channel (54 MiB),
+    on the 2048x1024 FP16 one (90 MiB)."""
+    from pylabolt_amd.engine import TWO_STEP_MIN_BYTES
+    from pylabolt_amd.solvers.fluidLB import Solver
+    for nx, ny, storage, want in ((1024, 256, "f64", False),
+                                  (2048, 1024, "f16dev", True)):
+        s = Solver(simulation=cases.poiseuille_channel(nx, ny),
+                   storage=storage, verbose=False)
+        s.set_backend()
+        e = s.engine
+        nbytes = 2 * e.f[0].numel() * e.f[0].element_size()
+        assert (nbytes > TWO_STEP_MIN_BYTES) == want
+        assert e.two_step == want, (nx, ny, storage, nbytes)
+        s.advance(6)
