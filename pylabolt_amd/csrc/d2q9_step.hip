@@ -115,15 +115,18 @@ __global__ __launch_bounds__(256) void k_step_all(
 // cell and two steps the lattice is read about 1.2 times (the grown rows
 // and columns, mostly L2 hits) and written once, against twice each for
 // two single steps.  Waves 0 .. DEEP_ROWS + 1 sweep the grown rows with the
-// single step's gather (lane l: cells j0 + CPL l ..); the last wave
-// computes the grown columns j0 - 1 and j0 + B of every grown row (lane
-// 2r + side).  Arithmetic per cell is that of the single step, so in FP64
+// single step's gather (lane l: cells j0 + CPL l ..); the last two waves
+// compute the grown columns, j0 - 1 and j0 + B, of every grown row (lane
+// r: row r).  Two waves of 14 scattered cells each finish stage one sooner
+// than one wave of 28 (measured: FP16 and BF16 steps 8 % faster, FP64
+// 0.5 %), and 16 waves still let two workgroups fill a CU.  Arithmetic per
+// cell is that of the single step, so in FP64
 // and FP32 storage the result is bitwise that of two single steps; in the
 // 16-bit forms the intermediate step stays in FP32 (LDS) instead of being
 // rounded to storage, and only the stored step is rounded.
 constexpr int DEEP_ROWS = 12;
 constexpr int DEEP_R1 = DEEP_ROWS + 2;
-constexpr int DEEP_WAVES = DEEP_R1 + 1;
+constexpr int DEEP_WAVES = DEEP_R1 + 2;
 
 template <int S, int MODEL, int FORCING>
 __device__ __forceinline__ void deep_collide(const d2q9_lattice& L,
@@ -163,11 +166,11 @@ __device__ __forceinline__ void deep_wall_global(const d2q9_lattice& L,
   }
 }
 
-// Two workgroups (30 waves) per CU need 8 waves per SIMD, i.e. at most 64
+// Two workgroups (32 waves) per CU need 8 waves per SIMD, i.e. at most 64
 // VGPRs.  The float forms are held to that (MRT with Guo forcing: 67-84
 // VGPRs unbounded; bounded, at most 16 bytes per lane spill, and the FP32
 // 4096x1024 MRT channel measured 41.4 us per step against 51.0).  FP64
-// keeps the default for a 960-thread block (4 waves per SIMD): its MRT with
+// keeps the default for a 1024-thread block (4 waves per SIMD): its MRT with
 // Guo forcing needs 94 VGPRs and, held to 64, spilled 196 bytes per lane
 // and ran 3x slower.
 template <int S, int MODEL, int FORCING>
@@ -217,9 +220,10 @@ __global__ __launch_bounds__(64 * DEEP_WAVES, Codec<S>::dev ? 8 : 4) void k_deep
 #pragma unroll
       for (int k = 0; k < 9; ++k) s1[w][k][1 + lane * CPL + q] = g[q][k];
     }
-  } else if (lane < 2 * DEEP_R1 && !((lane & 1) ? hi : lo)) {
-    // grown columns j0 - 1 and j0 + B (not the ghost column beside a wall)
-    const int r = lane >> 1, cl = (lane & 1) ? B + 1 : 0;
+  } else if (lane < DEEP_R1 && !((w - DEEP_R1) ? hi : lo)) {
+    // grown column j0 - 1 (wave DEEP_R1) or j0 + B (wave DEEP_R1 + 1), not
+    // the ghost column beside a wall
+    const int r = lane, cl = (w - DEEP_R1) ? B + 1 : 0;
     const int i = i0 - 1 + r, j = j0 - 1 + cl;
     T g[9];
 #pragma unroll
