@@ -196,11 +196,14 @@ int d2q9_stream_bc(const d2q9_lattice* L, const void* f_src, void* f_dst,
 
 /* The sweeps (d2q9_collide, d2q9_stream_bc, d2q9_fused_step) launch the
  * bulk tiles on `stream` and the general tiles plus fix-up cells on an
- * auxiliary stream of the same device, forked from and joined back to
- * `stream` with events (both passes write disjoint cells), so the two
- * overlap; the library creates that stream and its two events on first use
- * per device and keeps them for the process.  Work is ordered on `stream`
- * exactly as if it ran there alone, including under HIP graph capture.
+ * auxiliary stream, forked from and joined back to `stream` with events
+ * (both passes write disjoint cells), so the two overlap; the library
+ * creates one auxiliary stream and event pair per caller stream (and
+ * device) on first use and keeps them for the process (at most 4096
+ * caller streams).  Work is ordered on `stream` exactly as if it ran there
+ * alone, including under HIP graph capture, and host threads that drive
+ * different streams do not interact; calls on one stream from several
+ * threads must be ordered by the caller, as for any work on one stream.
  * Exception: a fused step in FP32 storage, or in FP16/BF16 storage with
  * n_general == 0, is one kernel on `stream`. */
 
@@ -371,6 +374,11 @@ int d2q9_adjoint_init_grad(const d2q9_lattice* L, const void* mu,
 
 /* Text of the last error on this thread. */
 const char* d2q9_last_error(void);
+
+/* Digest of the sources and compiler flags the library was built from ("h"
+ * + 32 hex digits); a loader compares it with its own tree's to refuse a
+ * stale library.  No reference counterpart.  No GPU needed. */
+const char* d2q9_source_hash(void);
 
 /* Struct layout for binding checks: sizeof(d2q9_lattice), sizeof of the two
  * boundary structs, then offsetof(d2q9_lattice, f) for f in plane, cpl,

@@ -121,6 +121,7 @@ SIGNATURES = {
     "d2q9_graph_launch": (ctypes.c_int, [_P, _P]),
     "d2q9_graph_destroy": (ctypes.c_int, [_P]),
     "d2q9_last_error": (ctypes.c_char_p, []),
+    "d2q9_source_hash": (ctypes.c_char_p, []),
     "d2q9_abi_layout": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int64),
                                        ctypes.c_int]),
     "d2q9_adjoint_step": (ctypes.c_int, [_LP, ctypes.c_int, _P, _P, _P, _P,
@@ -161,13 +162,29 @@ class NativeError(RuntimeError):
     pass
 
 
-def load_library(path=LIB_PATH):
-    """Load the library and declare every prototype (no GPU needed)."""
+def load_library(path=LIB_PATH, csrc=None, header=None):
+    """Load the library and declare every prototype (no GPU needed).
+
+    Refuses a library built from other sources than those of the tree it is
+    loaded from (``csrc`` and ``header`` default to the package's own): the
+    digest the build stamped into it must equal the tree's."""
+    from pylabolt_amd import build as _build
     if not os.path.exists(path):
         raise NativeError(
             f"{path} is missing: build it with `python -m pylabolt_amd.build`"
         )
+    csrc = _build.CSRC if csrc is None else csrc
+    header = _build.HEADER if header is None else header
     handle = ctypes.CDLL(path)
+    if not hasattr(handle, "d2q9_source_hash"):
+        raise NativeError(f"{path} carries no source digest: rebuild it")
+    handle.d2q9_source_hash.restype = ctypes.c_char_p
+    built = handle.d2q9_source_hash().decode()
+    want = "h" + _build.source_digest(csrc, header)
+    if built != want:
+        raise NativeError(
+            f"{path} is stale: built from sources with digest {built}, the "
+            f"tree has {want}; rebuild with `python -m pylabolt_amd.build`")
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(handle, name)
         fn.restype = res

@@ -20,12 +20,16 @@
 
 #include <cstddef>
 #include <initializer_list>
+#include <map>
+#include <memory>
 #include <mutex>
+#include <utility>
 #include <cstdio>
 #include <cstring>
 #include <string>
 
 #include "d2q9_device.h"
+#include "d2q9_reduce.h"
 #include "d2q9_sweep.h"
 
 using namespace d2q9;
@@ -354,57 +358,6 @@ __global__ __launch_bounds__(256) void k_moments(const d2q9_lattice L,
   mom[2 * P + c] = uy;
 }
 
-// ---- FP64 reductions (deterministic two-level tree) ----------------------
-constexpr int RED_THREADS = 256;
-constexpr int RED_MAX_BLOCKS = 2048;
-
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-  // wave-level tree then across the 4 waves of the workgroup
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) sh[wave] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) {
-    for (int w = 0; w < RED_THREADS / 64; ++w) r += sh[w];
-  }
-  __syncthreads();
-  return r;
-}
-
-template <int S>
-__global__ __launch_bounds__(RED_THREADS) void k_mass_partial(
-    const d2q9_lattice L, const typename Codec<S>::raw* f, double* partial) {
-  __shared__ double sh[RED_THREADS / 64];
-  const int64_t n = (int64_t)L.nx * L.ny;
-  double acc = 0.0;
-  for (int64_t e = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * RED_THREADS) {
-    const int i = 1 + (int)(e / L.ny), j = 1 + (int)(e % L.ny);
-    const int64_t c = cell_index(L, i, j);
-    if (!(L.flags[c] & D2Q9_CELL_FLUID)) continue;
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < 9; ++k)
-      s += Codec<S>::f64(f[pop_index<64 * Codec<S>::CPL>(L, i, j, k)], k, L);
-    acc += s;
-  }
-  double r = block_sum(acc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = r;
-}
-
-template <int NV>
-__global__ __launch_bounds__(RED_THREADS) void k_finish(const double* partial,
-                                                        int nblocks, double* out) {
-  __shared__ double sh[RED_THREADS / 64];
-  for (int q = 0; q < NV; ++q) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += RED_THREADS) acc += partial[q * RED_MAX_BLOCKS + b];
-    double r = block_sum(acc, sh);
-    if (threadIdx.x == 0) out[q] = r;
-  }
-}
-
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_residue_partial(
     const d2q9_lattice L, const T* mom, T* mom_old, double* partial) {
@@ -477,34 +430,46 @@ int validate(const d2q9_lattice* L) {
   return 0;
 }
 
-// One auxiliary stream and two events per device, created on first use and
-// kept for the life of the process (default priority).
+// One auxiliary stream and a fork/join event pair per caller stream (and
+// device), created on first use and kept for the life of the process
+// (default priority).  Keying them on the caller's stream means that two
+// host threads driving lattices on different streams never share an event:
+// each call's fork and join order only its own stream's work, and a graph
+// capture on one stream never pulls in another thread's eager work.
+// `order` makes the fork ... join sequence of one call atomic with respect
+// to other calls on the same caller stream.
 struct Aux {
   hipStream_t stream = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  std::mutex order;
 };
-constexpr int MAX_DEVICES = 64;
-Aux g_aux[MAX_DEVICES];
+constexpr size_t MAX_AUX = 4096;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<Aux>> g_aux;
 std::mutex g_aux_mutex;
 
-int aux_for_current_device(Aux** out) {
+int aux_for_stream(hipStream_t s, Aux** out) {
   int dev = 0;
   if (int r = check_hip(hipGetDevice(&dev), "hipGetDevice")) return r;
-  if (dev < 0 || dev >= MAX_DEVICES) return fail(D2Q9_E_ARG, "device index out of range");
   std::lock_guard<std::mutex> lock(g_aux_mutex);
-  Aux& a = g_aux[dev];
-  if (!a.stream) {
-    if (int r = check_hip(hipStreamCreateWithFlags(&a.stream, hipStreamNonBlocking),
+  std::unique_ptr<Aux>& slot = g_aux[std::make_pair(dev, s)];
+  if (!slot) {
+    if (g_aux.size() > MAX_AUX) {
+      g_aux.erase(std::make_pair(dev, s));
+      return fail(D2Q9_E_TOO_LARGE, "more than 4096 distinct caller streams");
+    }
+    std::unique_ptr<Aux> a(new Aux);
+    if (int r = check_hip(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking),
                           "hipStreamCreateWithFlags"))
       return r;
-    if (int r = check_hip(hipEventCreateWithFlags(&a.fork, hipEventDisableTiming),
+    if (int r = check_hip(hipEventCreateWithFlags(&a->fork, hipEventDisableTiming),
                           "hipEventCreateWithFlags"))
       return r;
-    if (int r = check_hip(hipEventCreateWithFlags(&a.join, hipEventDisableTiming),
+    if (int r = check_hip(hipEventCreateWithFlags(&a->join, hipEventDisableTiming),
                           "hipEventCreateWithFlags"))
       return r;
+    slot = std::move(a);
   }
-  *out = &a;
+  *out = slot.get();
   return 0;
 }
 
@@ -559,8 +524,10 @@ int launch_sweep(const d2q9_lattice* L, const void* fsrc, void* fdst,
   }
   hipStream_t aux = s;
   Aux* a = nullptr;
+  std::unique_lock<std::mutex> order;
   if (fork && L->n_bulk > 0 && L->n_general + nfixb > 0) {
-    if (int r = aux_for_current_device(&a)) return r;
+    if (int r = aux_for_stream(s, &a)) return r;
+    order = std::unique_lock<std::mutex>(a->order);
     aux = a->stream;
     if (int r = check_hip(hipEventRecord(a->fork, s), "hipEventRecord fork")) return r;
     if (int r = check_hip(hipStreamWaitEvent(aux, a->fork, 0), "hipStreamWaitEvent fork"))
@@ -627,17 +594,10 @@ int dispatch(const d2q9_lattice* L, const void* fsrc, void* fdst, const void* ms
   return fail(D2Q9_E_ARG, "bad storage");
 }
 
-int red_blocks(const d2q9_lattice* L) {
-  int64_t n = (int64_t)L->nx * L->ny;
-  int64_t b = (n + RED_THREADS * 8 - 1) / (RED_THREADS * 8);
-  if (b < 1) b = 1;
-  if (b > RED_MAX_BLOCKS) b = RED_MAX_BLOCKS;
-  return (int)b;
-}
-
 }  // namespace
 
 extern "C" {
+
 
 int d2q9_init_eq(const d2q9_lattice* L, void* f_a, void* f_b, const void* mom,
                  void* stream) {
@@ -708,7 +668,8 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
     return fail(D2Q9_E_ARG, "more deep tiles than the lattice holds");
   hipStream_t s = static_cast<hipStream_t>(stream);
   Aux* a = nullptr;
-  if (int r = aux_for_current_device(&a)) return r;
+  if (int r = aux_for_stream(s, &a)) return r;
+  std::lock_guard<std::mutex> order(a->order);
   if (int r = check_hip(hipEventRecord(a->fork, s), "hipEventRecord fork")) return r;
   if (n_deep > 0) {
     const dim3 grid((unsigned)n_deep), block(64, DEEP_WAVES);
@@ -804,23 +765,7 @@ int d2q9_mass_fp64(const d2q9_lattice* L, const void* f, double* work,
                    double* out, void* stream) {
   if (int r = validate(L)) return r;
   if (!f || !work || !out) return fail(D2Q9_E_ARG, "null buffer");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const int nb = red_blocks(L);
-  switch (L->storage) {
-#define D2Q9_MASS(SV)                                                          \
-  case SV:                                                                     \
-    hipLaunchKernelGGL((k_mass_partial<SV>), dim3(nb), dim3(RED_THREADS), 0,  \
-                       s, *L, static_cast<const Codec<SV>::raw*>(f), work);    \
-    break;
-    D2Q9_MASS(D2Q9_STORE_F64)
-    D2Q9_MASS(D2Q9_STORE_F32)
-    D2Q9_MASS(D2Q9_STORE_F16DEV)
-    D2Q9_MASS(D2Q9_STORE_BF16DEV)
-#undef D2Q9_MASS
-    default: return fail(D2Q9_E_ARG, "bad storage");
-  }
-  hipLaunchKernelGGL((k_finish<1>), dim3(1), dim3(RED_THREADS), 0, s, work, nb, out);
-  return check_hip(hipGetLastError(), "k_mass launch");
+  return launch_mass(L, f, work, out, static_cast<hipStream_t>(stream));
 }
 
 int d2q9_residues(const d2q9_lattice* L, const void* mom, void* mom_old,

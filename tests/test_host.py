@@ -238,6 +238,28 @@ def test_abi_layout_matches_ctypes_mirror():
     assert _native.abi_mismatches(lib) == []
 
 
+def test_stale_library_is_refused(tmp_path):
+    """The library carries the digest of the sources it was built from; a
+    tree whose sources differ (here: one byte of one kernel source) refuses
+    it, the unchanged tree loads it."""
+    import shutil
+    from pylabolt_amd import _native, build
+    csrc = tmp_path / "csrc"
+    shutil.copytree(build.CSRC, csrc,
+                    ignore=shutil.ignore_patterns("*.o", "*.so"))
+    header = tmp_path / "d2q9.h"
+    shutil.copy(build.HEADER, header)
+    _native.load_library(csrc=str(csrc), header=str(header))
+    src = csrc / "d2q9_step.hip"
+    src.write_text(src.read_text() + "\n// edited\n")
+    with pytest.raises(_native.NativeError, match="stale"):
+        _native.load_library(csrc=str(csrc), header=str(header))
+    src.write_text(src.read_text()[:-len("\n// edited\n")])
+    header.write_text(header.read_text().replace("cpl = 1", "cpl = 1 "))
+    with pytest.raises(_native.NativeError, match="stale"):
+        _native.load_library(csrc=str(csrc), header=str(header))
+
+
 def test_engine_refuses_without_gpu():
     import torch
     if torch.cuda.is_available():
