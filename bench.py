@@ -25,13 +25,22 @@ The CPU baseline is the C++/OpenMP restatement of the reference's
 multi-pass step (oracle/d2q9_ref.cpp) on the same workload for a bounded
 number of steps, on the host cores.
 
+The timed steps replay HIP graphs only: the graph chunk is chosen to
+divide ``--steps`` (a multiple of 4 up to 64; otherwise chunks of 32 and
+the remainder eager) and captured before the timer starts.
+
 Also reported: ``forms``, FP32 and 16-bit-storage runs of the same
-workload with their roofline fractions and velocity error against the
-FP64 run (relative L2 over the fluid cells, and RMS error over the case's
-reference speed u_ref = 0.05); ``batched``, 256 seeded lattices of uneven size advanced by one
-persistent launch per step; ``adjoint``, the forward + backward cost of the
-adjoint channel over its plain forward; and a plain copy kernel's bandwidth
-on the same device.
+workload with their roofline fractions and, after ``--long-steps`` steps
+of each (50,000, the case's length, by default), the velocity error
+against the FP64 run (relative L2 over the fluid cells, and RMS error over
+the case's reference speed u_ref = 0.05); ``channel``, the gravity-driven
+MRT + Guo channel 4096 x 1024 at its full 20,000 steps in every storage
+form: MLUPS and roofline of each, velocity error against the FP64 run and
+of every form against the start-up series of plane Poiseuille flow;
+``batched``, 256 seeded lattices of uneven size advanced by one persistent
+launch per step; ``adjoint``, the forward + backward cost of the adjoint
+channel over its plain forward (one graph of all its steps); and a plain
+copy kernel's bandwidth on the same device.
 """
 import argparse
 import json
@@ -74,10 +83,22 @@ def model_name(sim):
     return fl["model"] + (f"+{forcing}" if forcing else "")
 
 
+def graph_steps_for(steps):
+    """Graph chunk for ``steps`` timed steps: the largest multiple of 4 up
+    to 64 that divides it (every timed step then replays a graph), else
+    32."""
+    for g in range(64, 3, -4):
+        if steps % g == 0:
+            return g
+    return 32
+
+
 def time_engine(solver, steps, warmup, use_graph=True):
     import torch
     eng = solver.engine
     eng.advance(warmup, write_moments=False, use_graph=use_graph)
+    if use_graph:
+        eng.capture_graph()
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
@@ -137,7 +158,20 @@ def cpu_baseline(sim, target_s=15.0, max_steps=200):
     return {"value": nx * ny * n / dt / 1e6, "unit": "MLUPS",
             "cores": threads, "kind": "port",
             "sample": f"{n} steps of the same {nx}x{ny} workload after 2 "
-                      f"warm-up steps, OpenMP threads={threads}"}
+                      f"warm-up steps, OpenMP threads={threads}",
+            "host_cpu_count": os.cpu_count(), "host_cpu_model": cpu_model(),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def adjoint_cost(steps=2000, reps=3):
@@ -150,8 +184,6 @@ def adjoint_cost(steps=2000, reps=3):
     sim, F, u0, target = cases.adjoint_channel(steps=steps)
     nx, ny = F.shape[:2]
     out = {"workload": f"adjoint channel BGK+Guo FP64 {nx}x{ny}, {steps} steps"}
-    # plain forward: the same lattice, force field, graph-replayed steps
-    from pylabolt_amd.engine import D2Q9Engine, elements_from_boundary
     ro = {m: FlowRollout(sim, steps, checkpoint=m) for m in ("sqrt", "moments")}
     eng = ro["sqrt"].eng
     Ft = torch.tensor(F, device="cuda")
@@ -169,12 +201,34 @@ def adjoint_cost(steps=2000, reps=3):
             best = dt if best is None else min(best, dt)
         return best
 
+    # plain forward: one graph of the same init + N sweeps, moments stored
+    # at the last step only, no checkpoints; its buffers are set up by a
+    # rollout forward before the timer
+    r0 = ro["sqrt"]
+    r0.forward(Ft, ut)
+
+    def issue_plain():
+        from pylabolt_amd import _native as nt
+        nt.check(eng.lib.d2q9_init_eq(eng._Lp, nt.ptr(r0.work[0]),
+                                      nt.ptr(r0.work[1]), nt.ptr(r0.mom0),
+                                      eng._stream_ptr), "d2q9_init_eq")
+        prev = r0.work[0]
+        for n in range(1, steps + 1):
+            dst = r0.work[n % 2]
+            r0._sweep(prev, dst, r0.mscratch[(n - 1) % 2], r0.mscratch[n % 2],
+                      n == 1, n == steps)
+            prev = dst
+    g_plain = r0._capture(issue_plain)
+
     def plain():
-        eng.initialize()
-        eng.advance(steps, write_moments=True)
-        eng.synchronize()
+        from pylabolt_amd import _native as nt
+        nt.check(eng.lib.d2q9_graph_launch(g_plain, eng._stream_ptr),
+                 "graph launch")
     t_plain = timed(plain)
+    eng.lib.d2q9_graph_destroy(g_plain)
     out["plain_forward_ms"] = t_plain * 1e3
+    out["plain_forward"] = ("one HIP graph: d2q9_init_eq + collide + "
+                            f"{steps - 1} fused steps, no checkpoints")
     for m, r in ro.items():
         u = r.forward(Ft, ut)
         g = torch.zeros_like(u)
@@ -274,12 +328,101 @@ def sweep_rate(engine, bpc, ms_step, cells):
 
 
 def run_form(sim, storage, steps, warmup, tile, use_graph=True):
+    """Time ``steps`` steps of ``sim`` in one storage form after ``warmup``
+    steps.  Every form asks for the two-step sweep (FP64 and FP32 would
+    take it by size anyway; 16-bit storage only on request)."""
     from pylabolt_amd.solvers.fluidLB import Solver
     solver = Solver(simulation=sim, storage=storage, tile=tile,
-                    graph_steps=32, verbose=False)
+                    graph_steps=graph_steps_for(steps), verbose=False,
+                    two_step=True)
     solver.set_backend()
     ms, wall = time_engine(solver, steps, warmup, use_graph)
     return solver, ms, wall
+
+
+def timed_advance(engine, n):
+    """Advance ``n`` steps (moments stored at the last); GPU seconds by
+    HIP events on the engine's stream."""
+    import torch
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(engine.stream)
+    engine.advance(n, write_moments=True)
+    e1.record(engine.stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) * 1e-3
+
+
+def startup_series(y, H, nu, g, t, terms=400):
+    """Start-up of plane Poiseuille flow between walls at y = 0 and H
+    (uniform body force g from rest)."""
+    u_max = g * H * H / (8 * nu)
+    u = 4 * u_max * (y / H) * (1 - y / H)
+    for k in range(terms):
+        m = 2 * k + 1
+        u -= (32 * u_max / (np.pi ** 3 * m ** 3)) * np.sin(m * np.pi * y / H) \
+            * np.exp(-(m * np.pi / H) ** 2 * nu * t)
+    return u
+
+
+def channel_leg(storages, steps=None, warmup=100):
+    """Gravity-driven channel 4096 x 1024, MRT + second-order Guo, Re = 100
+    at u_max = 0.05, from rest, all its steps (20,000) in every storage
+    form, two-step sweeps from HIP graphs.  Per form: MLUPS and roofline of
+    the whole run, velocity error against the FP64 run (relative L2 over
+    all cells) and the x-averaged profile against the start-up series."""
+    import torch
+    from pylabolt_amd import cases
+    from pylabolt_amd.solvers.fluidLB import Solver
+    sim = cases.poiseuille_channel()
+    nx, ny = sim.mesh_dict["grid"]
+    steps = steps or sim.control_dict["end_time"]
+    nu = sim.transport_dict["kin_visc"]
+    g = sim.forcing_dict["gravity"][0]
+    y = np.arange(1, ny + 1) - 0.5
+    series = startup_series(y, ny, nu, g, steps)
+    out = {"workload": f"poiseuille channel D2Q9 {model_name(sim)} "
+                       f"{nx}x{ny}, {steps} steps from rest",
+           "steps": steps}
+    ref = None
+    for storage in storages:
+        s = Solver(simulation=sim, storage=storage, verbose=False,
+                   graph_steps=graph_steps_for(steps), two_step=True)
+        s.set_backend()
+        eng = s.engine
+        eng.capture_graph()
+        t = timed_advance(eng, steps)
+        ms_step = t * 1e3 / steps
+        bpc = BYTES_PER_CELL[storage]
+        achieved, single, two = sweep_rate(eng, bpc, ms_step, nx * ny)
+        _, vel = eng.moments_host()
+        u = vel.reshape(nx + 2, ny + 2, 2)[1:-1, 1:-1]
+        prof = u[:, :, 0].mean(axis=0)
+        rec = {"mlups": nx * ny / (ms_step * 1e-3) / 1e6,
+               "us_per_step": ms_step * 1e3, "gbs": achieved,
+               "frac": achieved / PEAK_GBS,
+               "single_step_equivalent_gbs": single,
+               "two_step": two is not None,
+               "u_rel_l2_vs_startup_series": float(
+                   np.linalg.norm(prof - series) / np.linalg.norm(series))}
+        if ref is None:
+            ref = u.copy()
+        else:
+            rec["u_rel_l2_vs_f64"] = float(np.linalg.norm(u - ref) /
+                                           np.linalg.norm(ref))
+            rec["u_rms_err_over_u_max"] = float(
+                np.sqrt(((u - ref) ** 2).sum(-1).mean()) / 0.05)
+            rec["speedup_vs_f64"] = out["f64"]["us_per_step"] / (ms_step * 1e3)
+        out[storage] = rec
+        log(f"channel {storage}: {ms_step * 1e3:.1f} us/step "
+            f"{rec['mlups']:.0f} MLUPS ({rec['frac']:.3f} of peak), u vs "
+            f"series {rec['u_rel_l2_vs_startup_series']:.2e}"
+            + (f", vs f64 {rec['u_rel_l2_vs_f64']:.2e}"
+               if "u_rel_l2_vs_f64" in rec else ""))
+        del s, eng
+        torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -310,6 +453,10 @@ def run(real_stdout):
     ap.add_argument("--no-copy", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-channel", action="store_true")
+    ap.add_argument("--long-steps", type=int, default=50000,
+                    help="cylinder forms run on to this step count for "
+                    "their velocity error (0: not)")
     ap.add_argument("--profile-run", action="store_true",
                     help="a short run for rocprofv3: main kernel + copy "
                     "(--workload batched: the batched sweep only)")
@@ -344,52 +491,63 @@ def run(real_stdout):
         return None
 
     log(f"workload {args.workload} {nx}x{ny} storage={args.storage} "
-        f"tile={tile} steps={args.steps} warmup={args.warmup}")
-    solver, ms, wall = run_form(sim, args.storage, args.steps, args.warmup,
-                                tile)
-    ms_step = ms / args.steps
-    mlups = cells / (ms_step * 1e-3) / 1e6
-    bpc = BYTES_PER_CELL[args.storage]
-    achieved, single_equiv, two = sweep_rate(solver.engine, bpc, ms_step,
-                                             cells)
-    log(f"{args.storage}: {ms_step * 1e3:.1f} us/step  {mlups:.0f} MLUPS  "
-        f"{achieved:.0f} GB/s ({achieved / PEAK_GBS:.3f} of peak), wall "
-        f"{wall:.2f}s")
-    ref_rho, ref_vel = None, None
-    solver.engine.advance(1, write_moments=True)
-    ref_rho, ref_vel = solver.engine.moments_host()
-    fluid = solver.engine.tables.fluid_pad.ravel()
-    steps_done = solver.engine.time_step
-    del solver
-    torch.cuda.empty_cache()
-
-    forms = {args.storage: {"mlups": mlups, "us_per_step": ms_step * 1e3,
-                            "gbs": achieved, "frac": achieved / PEAK_GBS}}
-    for form in [f for f in args.forms.split(",") if f and f != args.storage]:
-        s2, ms2, _ = run_form(sim, form, args.steps, args.warmup, tile)
+        f"tile={tile} steps={args.steps} warmup={args.warmup} "
+        f"graph chunk {graph_steps_for(args.steps)}")
+    long_steps = args.long_steps if args.workload == "cylinder" else 0
+    forms = {}
+    ref_vel = None
+    main = None
+    storages = [args.storage] + [f for f in args.forms.split(",")
+                                 if f and f != args.storage]
+    for form in storages:
+        s2, ms2, wall2 = run_form(sim, form, args.steps, args.warmup, tile)
         st2 = ms2 / args.steps
         m2 = cells / (st2 * 1e-3) / 1e6
         a2, single2, two2 = sweep_rate(s2.engine, BYTES_PER_CELL[form], st2,
                                        cells)
-        s2.engine.advance(steps_done - s2.engine.time_step, write_moments=True)
-        _, vel2 = s2.engine.moments_host()
-        du = vel2[fluid] - ref_vel[fluid]
-        l2 = float(np.sqrt((du ** 2).sum() / max((ref_vel[fluid] ** 2).sum(),
-                                                 1e-300)))
-        rms_ref = float(np.sqrt((du ** 2).sum(axis=-1).mean())) / U_REF
-        forms[form] = {"mlups": m2, "us_per_step": st2 * 1e3, "gbs": a2,
-                       "frac": a2 / PEAK_GBS,
-                       "two_step": two2 is not None,
-                       "single_step_equivalent_gbs": single2,
-                       "speedup_vs_" + args.storage: m2 / mlups,
-                       "u_rel_l2_vs_" + args.storage: l2,
-                       "u_rms_err_over_u_ref": rms_ref,
-                       "u_rms_" + args.storage: float(np.sqrt(
-                           (ref_vel[fluid] ** 2).sum(axis=-1).mean())),
-                       "steps_compared": steps_done}
-        log(f"{form}: {st2 * 1e3:.1f} us/step {m2:.0f} MLUPS {a2:.0f} GB/s "
-            f"u L2 rel err {l2:.2e}, rms err / u_ref {rms_ref:.2e}")
+        rec = {"mlups": m2, "us_per_step": st2 * 1e3, "gbs": a2,
+               "frac": a2 / PEAK_GBS, "two_step": two2 is not None,
+               "single_step_equivalent_gbs": single2}
+        if main is None:
+            main = (ms2, wall2, st2, m2, a2, single2, two2)
+            log(f"{form}: {st2 * 1e3:.1f} us/step  {m2:.0f} MLUPS  "
+                f"{a2:.0f} GB/s ({a2 / PEAK_GBS:.3f} of peak), wall "
+                f"{wall2:.2f}s")
+        else:
+            rec["speedup_vs_" + args.storage] = m2 / main[3]
+            log(f"{form}: {st2 * 1e3:.1f} us/step {m2:.0f} MLUPS {a2:.0f} "
+                f"GB/s ({a2 / PEAK_GBS:.3f} of peak)")
+        if long_steps:
+            eng = s2.engine
+            n_long = long_steps - eng.time_step
+            t_long = timed_advance(eng, n_long)
+            _, vel2 = eng.moments_host()
+            fluid = eng.tables.fluid_pad.ravel()
+            rec["long_run"] = {"steps": long_steps,
+                               "us_per_step": t_long * 1e6 / n_long}
+            if ref_vel is None:
+                ref_vel = vel2
+                rec["long_run"]["u_rms"] = float(np.sqrt(
+                    (vel2[fluid] ** 2).sum(axis=-1).mean()))
+            else:
+                du = vel2[fluid] - ref_vel[fluid]
+                rec["long_run"]["u_rel_l2_vs_" + args.storage] = float(
+                    np.sqrt((du ** 2).sum() / max(
+                        (ref_vel[fluid] ** 2).sum(), 1e-300)))
+                rec["long_run"]["u_rms_err_over_u_ref"] = float(
+                    np.sqrt((du ** 2).sum(axis=-1).mean())) / U_REF
+                log(f"{form} after {long_steps} steps: u rel L2 vs "
+                    f"{args.storage} "
+                    f"{rec['long_run']['u_rel_l2_vs_' + args.storage]:.2e}")
+        forms[form] = rec
         del s2
+        torch.cuda.empty_cache()
+    ms, wall, ms_step, mlups, achieved, single_equiv, two = main
+    bpc = BYTES_PER_CELL[args.storage]
+
+    chan = None
+    if not args.no_channel:
+        chan = channel_leg(storages)
         torch.cuda.empty_cache()
 
     bat = None
@@ -413,6 +571,10 @@ def run(real_stdout):
     traffic = pmc["hbm_bytes_per_launch"] if pmc else None
     if pmc and two is not None and pmc.get("launch") != "two-step":
         traffic = None     # counters of another launch shape
+    traffic_source = (f"profiles/pmc_{args.workload}_{args.storage}.json: "
+                      "rocprofv3 FETCH_SIZE and WRITE_SIZE of the same "
+                      "launch, scaled by a copy kernel's counters "
+                      "(not measured in this run)" if traffic else None)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -440,6 +602,7 @@ def run(real_stdout):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_GBS,
                      "traffic": traffic,
+                     "traffic_source": traffic_source,
                      "bytes_per_cell_update": (
                          two["algorithmic_bytes_per_launch"] / (2 * cells)
                          if two else bpc),
@@ -447,6 +610,7 @@ def run(real_stdout):
                      "single_step_equivalent_gbs": single_equiv},
         "cpu_baseline": cpu,
         "forms": forms,
+        "channel": chan,
         "batched": bat,
         "adjoint": adj,
         "copy_kernel_gbs": copy_gbs,

@@ -31,7 +31,9 @@ engine identically (parity for them is against this definition):
   reference's moment matrix M and ``S = (1,1,1,1,1,1,1,omega,omega)``
   (base/collision_operator.py:136-165, whose own setup is broken); Fbar is
   the Guo source of the selected forcing without its (1 - omega/2) factor.
-  The operation order is spelled out in ``mrt_collide``.
+  It is evaluated as ``feq + M^-1 [(I - S) M (f - feq) + (I - S/2) M Fbar]``
+  over the opposite-direction pairs; the operation order is spelled out in
+  ``mrt_collide``, ``pair_moments`` and ``pair_inverse``.
 * zero_gradient: the unknown directions of a boundary node take the values
   the streaming pass produced in the same directions at the inward
   neighbour (before any boundary element ran).
@@ -212,60 +214,101 @@ def bgk_collide(case):
     case.pop[m] = out
 
 
-def mrt_moments(vals):
-    """m_i = sum_k M[i,k] * v_k, accumulated in k order from 0.0."""
-    n = vals.shape[0]
-    m = np.zeros((n, 9), dtype=np.float64)
-    for i in range(9):
-        acc = np.zeros(n, dtype=np.float64)
-        for k in range(9):
-            if M_MRT[i, k] != 0:
-                acc = acc + np.float64(M_MRT[i, k]) * vals[:, k]
-        m[:, i] = acc
-    return m
+# opposite pairs (k, inv k) in the order the MRT form below visits them
+PAIRS = ((1, 3), (2, 4), (5, 7), (6, 8))
+
+
+def pair_moments(x0, S, D):
+    """Rows of M applied to a population-like vector given as its rest value
+    ``x0`` and the sums ``S[n]`` and differences ``D[n]`` of the pairs
+    ``PAIRS[n]`` (the columns of k and inv k agree on rows 0, 1, 2, 7, 8
+    and are opposite on rows 3..6), in this operation order."""
+    s12 = S[0] + S[1]
+    s34 = S[2] + S[3]
+    a = D[2] - D[3]
+    b = D[2] + D[3]
+    return [(x0 + s12) + s34,
+            ((-4.0 * x0) - s12) + 2.0 * s34,
+            ((4.0 * x0) + s34) - 2.0 * s12,
+            D[0] + a,
+            a - 2.0 * D[0],
+            D[1] + b,
+            b - 2.0 * D[1],
+            S[0] - S[1],
+            S[2] - S[3]]
+
+
+def pair_inverse(gm):
+    """``M^T gm`` by columns k = 0..8, in this operation order (with
+    ``gm = diag(1/|row|^2) p`` it is ``M^-1 p``)."""
+    out0 = gm[0] + 4.0 * (gm[2] - gm[1])
+    base13 = (gm[0] - gm[1]) - 2.0 * gm[2]
+    sym13 = base13 + gm[7]
+    sym24 = base13 - gm[7]
+    asym13 = gm[3] - 2.0 * gm[4]
+    asym24 = gm[5] - 2.0 * gm[6]
+    base57 = (gm[0] + gm[2]) + 2.0 * gm[1]
+    sym57 = base57 + gm[8]
+    sym68 = base57 - gm[8]
+    p34 = gm[3] + gm[4]
+    p56 = gm[5] + gm[6]
+    asym57 = p34 + p56
+    asym68 = p56 - p34
+    return [out0, sym13 + asym13, sym24 + asym24, sym13 - asym13,
+            sym24 - asym24, sym57 + asym57, sym68 + asym68, sym57 - asym57,
+            sym68 - asym68]
 
 
 def mrt_collide(case, rates=None):
-    """Build-defined MRT with Guo forcing (see module docstring).
+    """Build-defined MRT with Guo forcing (see module docstring), in the
+    opposite-pair form:
 
-    With ``s = rates``:
-      m  = M f, meq = M feq, mF = M Fbar          (mrt_moments order)
-      d_i = s_i * (m_i - meq_i) - (1 - 0.5 * s_i) * mF_i
-      g_i = d_i * (1 / |row_i|^2)
-      f*_k = f_k - (sum over i in order of M[i,k] * g_i, from 0.0)
-    """
+      e_k = f_k - feq_k,  Fbar_k = Guo source without (1 - omega/2)
+      me = M e, mf = M Fbar         (pair_moments of the pair sums and
+                                      differences e_k + e_kb, e_k - e_kb)
+      p_i = (1 - s_i) me_i + (1 - 0.5 s_i) mf_i   (no forcing: first term)
+      f*_k = feq_k + (M^-1 p)_k     (pair_inverse of p_i * (1/|row_i|^2))
+
+    which is f - M^-1 [S (M f - M feq) - (I - S/2) M Fbar] in exact
+    arithmetic; ``s = rates`` (default (1,1,1,1,1,1,1,omega,omega))."""
     m_ = case.fluid
     c = case.const
     om = case.omega
     if rates is None:
         rates = np.array([1, 1, 1, 1, 1, 1, 1, om, om], dtype=np.float64)
+    rates = np.asarray(rates, dtype=np.float64)
     rho = case.density[m_]
     ux, uy = case.velocity[m_, 0], case.velocity[m_, 1]
     fx, fy = case.force_field[m_, 0], case.force_field[m_, 1]
     pn = case.pop_new[m_]
-    n = pn.shape[0]
-    feq = np.empty((n, 9))
-    fbar = np.zeros((n, 9))
-    for k in range(9):
-        feq[:, k] = equilibrium(rho, ux, uy, k, c)
-        if case.forcing is not None:
-            fbar[:, k] = guo_source(k, ux, uy, fx, fy, case.forcing, c)
-    mm = mrt_moments(pn)
-    meq = mrt_moments(feq)
-    mf = mrt_moments(fbar)
-    g = np.empty((n, 9))
+    forced = case.forcing is not None
+    feq = [equilibrium(rho, ux, uy, k, c) for k in range(9)]
+    fb = [guo_source(k, ux, uy, fx, fy, case.forcing, c) if forced else None
+          for k in range(9)]
+    e0 = pn[:, 0] - feq[0]
+    eS, eD, fS, fD = [], [], [], []
+    for k, kb in PAIRS:
+        ek = pn[:, k] - feq[k]
+        ekb = pn[:, kb] - feq[kb]
+        eS.append(ek + ekb)
+        eD.append(ek - ekb)
+        if forced:
+            fS.append(fb[k] + fb[kb])
+            fD.append(fb[k] - fb[kb])
+    me = pair_moments(e0, eS, eD)
+    mf = pair_moments(fb[0], fS, fD) if forced else None
+    gm = []
     for i in range(9):
-        s = np.float64(rates[i])
-        d = s * (mm[:, i] - meq[:, i]) - (1 - 0.5 * s) * mf[:, i]
-        g[:, i] = d * np.float64(1.0 / MRT_ROW_NORM[i])
-    out = np.empty_like(pn)
+        s = rates[i]
+        p = (1.0 - s) * me[i]
+        if forced:
+            p = p + (1.0 - 0.5 * s) * mf[i]
+        gm.append(p * np.float64(1.0 / MRT_ROW_NORM[i]))
+    out = pair_inverse(gm)
+    res = np.empty_like(pn)
     for k in range(9):
-        acc = np.zeros(n)
-        for i in range(9):
-            if M_MRT[i, k] != 0:
-                acc = acc + np.float64(M_MRT[i, k]) * g[:, i]
-        out[:, k] = pn[:, k] - acc
-    case.pop[m_] = out
+        res[:, k] = feq[k] + out[k]
+    case.pop[m_] = res
 
 
 def periodic_wrap(case):
@@ -441,6 +484,57 @@ def fluid_boundary_nodes(shape, solid, ghost, solid_id):
                 sid[ind] = sid[nb]
                 break
     return fb, sid
+
+
+def obstacle_boundary_fields(shape, solid, ghost, solid_id, circles,
+                             x_periodic, y_periodic):
+    """compute_obstacle_boundary (parallel/cpu/obstacle_kernels.py:307-348,
+    both branches) then compute_normals_circle (:97-143) for each circle
+    ``(id, center)``, sequential, single rank (offset 0).  Returns
+    (solid_boundary, fluid_boundary, solid_id, surface_normals) over the
+    padded lattice."""
+    nxp, nyp = shape
+    Nx, Ny = nxp - 2, nyp - 2
+    size = nxp * nyp
+    sb = np.zeros(size, dtype=bool)
+    fb = np.zeros(size, dtype=bool)
+    sid = np.asarray(solid_id).copy()
+    for ind in range(size):
+        if ghost[ind]:
+            continue
+        x, y = divmod(ind, nyp)
+        for k in range(9):
+            nb = (x + CX[k]) * nyp + (y + CY[k])
+            if solid[ind] and not solid[nb]:
+                sb[ind] = True
+                break
+            if not solid[ind] and solid[nb]:
+                fb[ind] = True
+                sid[ind] = sid[nb]
+                break
+    normals = np.zeros((size, 2), dtype=np.float64)
+    for cid, center in circles:
+        for ind in range(size):
+            if sid[ind] != cid or not (sb[ind] or fb[ind]):
+                continue
+            x, y = divmod(ind, nyp)
+            rx = (x - 1) - center[0]
+            ry = (y - 1) - center[1]
+            rxm, rym = rx, ry
+            if x_periodic:
+                if abs(rx + Nx) < abs(rxm):
+                    rxm = rx + Nx
+                if abs(rx - Nx) < abs(rxm):
+                    rxm = rx - Nx
+            if y_periodic:
+                if abs(ry + Ny) < abs(rym):
+                    rym = ry + Ny
+                if abs(ry - Ny) < abs(rym):
+                    rym = ry - Ny
+            mag = np.sqrt(rxm * rxm + rym * rym)
+            normals[ind, 0] = rxm / mag
+            normals[ind, 1] = rym / mag
+    return sb, fb, sid, normals
 
 
 def obstacle_force_torque(case, solid_id, fluid_boundary, ref_point,

@@ -19,12 +19,8 @@ namespace {
 const int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 const int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 const int INV[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
-const int MM[9][9] = {
-    {1, 1, 1, 1, 1, 1, 1, 1, 1},   {-4, -1, -1, -1, -1, 2, 2, 2, 2},
-    {4, -2, -2, -2, -2, 1, 1, 1, 1}, {0, 1, 0, -1, 0, 1, -1, -1, 1},
-    {0, -2, 0, 2, 0, 1, -1, -1, 1}, {0, 0, 1, 0, -1, 1, 1, -1, -1},
-    {0, 0, -2, 0, 2, 1, 1, -1, -1}, {0, 1, -1, 1, -1, 0, 0, 0, 0},
-    {0, 0, 0, 0, 0, 1, -1, 1, -1}};
+// squared row norms of the MRT moment matrix (base/collision_operator.py
+// :147-157), whose rows are orthogonal: M^-1 = M^T diag(1 / ROWNORM)
 const int ROWNORM[9] = {9, 36, 36, 6, 12, 6, 12, 4, 4};
 }  // namespace
 
@@ -82,6 +78,79 @@ static inline double guo_k(const oracle_case* c, int k, double ux, double uy,
   double const_x = ((double)CX[k] - ux) * c->inv_cs_2 + (cu * (double)CX[k]) * c->inv_cs_4;
   double const_y = ((double)CY[k] - uy) * c->inv_cs_2 + (cu * (double)CY[k]) * c->inv_cs_4;
   return c->w[k] * (const_x * fx + const_y * fy);
+}
+
+// The nine moments (rows of M) of a population-like vector from its rest
+// value x0 and the sums S and differences D of the pairs (1,3), (2,4),
+// (5,7), (6,8); operation order of d2q9_oracle.pair_moments.
+static void pair_moments(double x0, const double* S, const double* D, double* m) {
+  const double s12 = S[0] + S[1], s34 = S[2] + S[3];
+  const double a = D[2] - D[3], b = D[2] + D[3];
+  m[0] = (x0 + s12) + s34;
+  m[1] = ((-4.0 * x0) - s12) + 2.0 * s34;
+  m[2] = ((4.0 * x0) + s34) - 2.0 * s12;
+  m[3] = D[0] + a;
+  m[4] = a - 2.0 * D[0];
+  m[5] = D[1] + b;
+  m[6] = b - 2.0 * D[1];
+  m[7] = S[0] - S[1];
+  m[8] = S[2] - S[3];
+}
+
+// M^T gm by columns; operation order of d2q9_oracle.pair_inverse.
+static void pair_inverse(const double* gm, double* out) {
+  out[0] = gm[0] + 4.0 * (gm[2] - gm[1]);
+  const double base13 = (gm[0] - gm[1]) - 2.0 * gm[2];
+  const double sym13 = base13 + gm[7], sym24 = base13 - gm[7];
+  const double asym13 = gm[3] - 2.0 * gm[4], asym24 = gm[5] - 2.0 * gm[6];
+  const double base57 = (gm[0] + gm[2]) + 2.0 * gm[1];
+  const double sym57 = base57 + gm[8], sym68 = base57 - gm[8];
+  const double p34 = gm[3] + gm[4], p56 = gm[5] + gm[6];
+  const double asym57 = p34 + p56, asym68 = p56 - p34;
+  out[1] = sym13 + asym13;
+  out[2] = sym24 + asym24;
+  out[3] = sym13 - asym13;
+  out[4] = sym24 - asym24;
+  out[5] = sym57 + asym57;
+  out[6] = sym68 + asym68;
+  out[7] = sym57 - asym57;
+  out[8] = sym68 - asym68;
+}
+
+// Build-defined MRT with Guo forcing, opposite-pair form of
+// d2q9_oracle.mrt_collide: f* = feq + M^-1 [(I - S) M (f - feq) +
+// (I - S/2) M Fbar], S = (1,1,1,1,1,1,1,omega,omega).
+static void mrt_collide(const oracle_case* c, const double* pn, double* po, double rho,
+                        double ux, double uy, double fx, double fy) {
+  static const int PK[4] = {1, 2, 5, 6}, PB[4] = {3, 4, 7, 8};
+  const double om = c->omega;
+  const bool forced = c->forcing != 0;
+  double feq[9], fb[9];
+  for (int k = 0; k < 9; ++k) {
+    feq[k] = feq_k(c, k, rho, ux, uy);
+    fb[k] = forced ? guo_k(c, k, ux, uy, fx, fy) : 0.0;
+  }
+  const double e0 = pn[0] - feq[0];
+  double eS[4], eD[4], fS[4], fD[4];
+  for (int n = 0; n < 4; ++n) {
+    const double ek = pn[PK[n]] - feq[PK[n]];
+    const double ekb = pn[PB[n]] - feq[PB[n]];
+    eS[n] = ek + ekb;
+    eD[n] = ek - ekb;
+    fS[n] = fb[PK[n]] + fb[PB[n]];
+    fD[n] = fb[PK[n]] - fb[PB[n]];
+  }
+  double me[9], mf[9], gm[9], out[9];
+  pair_moments(e0, eS, eD, me);
+  pair_moments(fb[0], fS, fD, mf);
+  for (int i = 0; i < 9; ++i) {
+    const double s = (i >= 7) ? om : 1.0;
+    double p = (1.0 - s) * me[i];
+    if (forced) p = p + (1.0 - 0.5 * s) * mf[i];
+    gm[i] = p * (1.0 / (double)ROWNORM[i]);
+  }
+  pair_inverse(gm, out);
+  for (int k = 0; k < 9; ++k) po[k] = feq[k] + out[k];
 }
 
 int oracle_initialize(oracle_case* c) {
@@ -149,34 +218,7 @@ static void moments_and_collide(oracle_case* c) {
         po[k] = v;
       }
     } else {
-      double feq[9], fb[9], mm[9], me[9], mf[9], g[9];
-      for (int k = 0; k < 9; ++k) {
-        feq[k] = feq_k(c, k, rho, ux, uy);
-        fb[k] = c->forcing ? guo_k(c, k, ux, uy, fx, fy) : 0.0;
-      }
-      for (int i = 0; i < 9; ++i) {
-        double a = 0.0, b = 0.0, d = 0.0;
-        for (int k = 0; k < 9; ++k) {
-          if (MM[i][k] == 0) continue;
-          a = a + (double)MM[i][k] * pn[k];
-          b = b + (double)MM[i][k] * feq[k];
-          d = d + (double)MM[i][k] * fb[k];
-        }
-        mm[i] = a; me[i] = b; mf[i] = d;
-      }
-      for (int i = 0; i < 9; ++i) {
-        double s = (i >= 7) ? om : 1.0;
-        double d = s * (mm[i] - me[i]) - (1.0 - 0.5 * s) * mf[i];
-        g[i] = d * (1.0 / (double)ROWNORM[i]);
-      }
-      for (int k = 0; k < 9; ++k) {
-        double acc = 0.0;
-        for (int i = 0; i < 9; ++i) {
-          if (MM[i][k] == 0) continue;
-          acc = acc + (double)MM[i][k] * g[i];
-        }
-        po[k] = pn[k] - acc;
-      }
+      mrt_collide(c, pn, po, rho, ux, uy, fx, fy);
     }
   }
 }

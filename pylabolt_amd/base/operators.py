@@ -52,6 +52,70 @@ def obstacle_boundary_nodes(state):
     return nodes, mask[ii[sel], jj[sel]], first[ii[sel], jj[sel]]
 
 
+def mark_obstacle_boundaries(state):
+    """Boundary fields of the obstacles, as the reference's ObstacleOperator
+    leaves them at set-up (obstacle_operator.py:36-45): the kernels
+    compute_obstacle_boundary and compute_normals_circle
+    (parallel/cpu/obstacle_kernels.py:97-143, 307-348), vectorised.
+
+    * ``solid_boundary``: a non-ghost solid node with a non-solid neighbour
+      at (i + c_k);
+    * ``fluid_boundary``: a non-ghost fluid node with a solid neighbour; it
+      takes the ``solid_id`` of the first one in direction order;
+    * ``surface_normals``: on both kinds of node of a circle, the unit
+      vector from the centre to the node (global node coordinates i - 1,
+      j - 1, minimum image across periodic sides).
+
+    Raises RuntimeError when two obstacles share a fluid boundary node
+    (check_fluid_boundary_overlap, obstacle_operator.py:156-185)."""
+    nxp, nyp = (int(v) for v in state.domain.shape)
+    f = state.fields
+    solid = np.asarray(f.solid, bool).reshape(nxp, nyp)
+    ghost = np.asarray(f.ghost_node, bool).reshape(nxp, nyp)
+    sid = f.solid_id.reshape(nxp, nyp)
+    sb = f.solid_boundary.reshape(nxp, nyp)
+    fb = f.fluid_boundary.reshape(nxp, nyp)
+    ii, jj = np.nonzero(~ghost)
+    sol = solid[ii, jj]
+    si, sj = ii[sol], jj[sol]
+    fi, fj = ii[~sol], jj[~sol]
+    done = np.zeros(fi.size, dtype=bool)
+    take_id = np.full(fi.size, -1, dtype=sid.dtype)
+    for k in range(9):
+        sb[si, sj] |= ~solid[si + CX[k], sj + CY[k]]
+        hit = solid[fi + CX[k], fj + CY[k]] & ~done
+        take_id[hit] = sid[fi[hit] + CX[k], fj[hit] + CY[k]]
+        done |= hit
+    fb[fi[done], fj[done]] = True
+    sid[fi[done], fj[done]] = take_id[done]
+    # overlap: a fluid boundary node with a neighbour of another obstacle
+    bi, bj = np.nonzero(fb & ~ghost)
+    for k in range(9):
+        nb = sid[bi + CX[k], bj + CY[k]]
+        if np.any((nb != sid[bi, bj]) & (nb != -1)):
+            raise RuntimeError(
+                "Fluid Boundary node overlap detected: two solid obstacles "
+                "have a common fluid boundary node")
+    nrm = f.surface_normals.reshape(nxp, nyp, 2)
+    Nx, Ny = (int(v) for v in state.mesh.grid_global_shape)
+    off = state.domain.offset
+    for o in state.obstacle.obstacles:
+        if o.type != "circle":
+            continue
+        oi, oj = np.nonzero((sid == o.id) & (sb | fb))
+        rx = (oi - 1 + off[0]) - o.center[0]
+        ry = (oj - 1 + off[1]) - o.center[1]
+        if state.boundary.x_periodic:
+            rx = np.where(np.abs(rx + Nx) < np.abs(rx), rx + Nx, rx)
+            rx = np.where(np.abs(rx - Nx) < np.abs(rx), rx - Nx, rx)
+        if state.boundary.y_periodic:
+            ry = np.where(np.abs(ry + Ny) < np.abs(ry), ry + Ny, ry)
+            ry = np.where(np.abs(ry - Ny) < np.abs(ry), ry - Ny, ry)
+        mag = np.sqrt(rx * rx + ry * ry)
+        nrm[oi, oj, 0] = rx / mag
+        nrm[oi, oj, 1] = ry / mag
+
+
 def wall_boundary_groups(state):
     """One group per wall element: its non-solid nodes and the out_list
     directions (compute_boundary_force_single_phase,
@@ -172,6 +236,7 @@ class ObstacleOperator:
     def __init__(self, model, state, backend=None, mpi_operator=None,
                  force_operator=None, verbose=True):
         self.no_of_obstacles = state.obstacle.no_of_obstacles
+        mark_obstacle_boundaries(state)
 
     def set_backend(self, state, backend, verbose=True):
         return None

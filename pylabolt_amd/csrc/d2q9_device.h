@@ -580,50 +580,114 @@ __device__ __forceinline__ void collide_mrt_pairs(float (&g)[9], const Consts<fl
   g[8] -= sym68 - asym68;
 }
 
-// MRT with Guo forcing in place (FP64 operation order of oracle
-// mrt_collide).  The moment map is linear and M w is a constant, so the
-// same code applies to deviations (m - meq is unchanged).
+// Moments (rows of M) of a population-like vector from its rest value x0
+// and the sums S and differences D of the pairs (1,3), (2,4), (5,7),
+// (6,8), without fused multiply-adds: the FP64 operation order of oracle
+// pair_moments.
+template <typename T>
+__device__ __forceinline__ void pair_moments(T x0, const T (&S)[4], const T (&D)[4],
+                                             T (&m)[9]) {
+  const T s12 = S[0] + S[1], s34 = S[2] + S[3];
+  const T a = D[2] - D[3], b = D[2] + D[3];
+  m[0] = (x0 + s12) + s34;
+  m[1] = (((T)-4 * x0) - s12) + (T)2 * s34;
+  m[2] = (((T)4 * x0) + s34) - (T)2 * s12;
+  m[3] = D[0] + a;
+  m[4] = a - (T)2 * D[0];
+  m[5] = D[1] + b;
+  m[6] = b - (T)2 * D[1];
+  m[7] = S[0] - S[1];
+  m[8] = S[2] - S[3];
+}
+
+// M^T gm by columns (oracle pair_inverse).
+template <typename T>
+__device__ __forceinline__ void pair_inverse(const T (&gm)[9], T (&out)[9]) {
+  out[0] = gm[0] + (T)4 * (gm[2] - gm[1]);
+  const T base13 = (gm[0] - gm[1]) - (T)2 * gm[2];
+  const T sym13 = base13 + gm[7], sym24 = base13 - gm[7];
+  const T asym13 = gm[3] - (T)2 * gm[4], asym24 = gm[5] - (T)2 * gm[6];
+  const T base57 = (gm[0] + gm[2]) + (T)2 * gm[1];
+  const T sym57 = base57 + gm[8], sym68 = base57 - gm[8];
+  const T p34 = gm[3] + gm[4], p56 = gm[5] + gm[6];
+  const T asym57 = p34 + p56, asym68 = p56 - p34;
+  out[1] = sym13 + asym13;
+  out[2] = sym24 + asym24;
+  out[3] = sym13 - asym13;
+  out[4] = sym24 - asym24;
+  out[5] = sym57 + asym57;
+  out[6] = sym68 + asym68;
+  out[7] = sym57 - asym57;
+  out[8] = sym68 - asym68;
+}
+
+// MRT with Guo forcing in place, FP64 storage: the operation order of
+// oracle mrt_collide,
+//   e = g - feq, Fbar = Guo source (pair sums and differences),
+//   p = (I - S) M e + (I - S/2) M Fbar,  g = feq + M^-1 p,
+// which equals g - M^-1 [S (M g - M feq) - (I - S/2) M Fbar].  The
+// populations are consumed pair by pair and feq is evaluated again for the
+// result instead of being held across the moments (the compiler is shown
+// copies of rho and u it cannot see through, so it cannot merge the two
+// evaluations): the live set stays near 24 doubles, which keeps the
+// two-step kernel at two workgroups per CU.
 template <typename T, int FORCING, bool DEV>
 __device__ __forceinline__ void collide_mrt(T (&g)[9], const Consts<T>& K,
                                             T drho, T rho, T ux, T uy, T fx, T fy) {
+  static_assert(!DEV, "deviation storage forms use collide_mrt_pairs");
+  constexpr bool forced = FORCING != D2Q9_FORCING_NONE;
+  constexpr int PK[4] = {1, 2, 5, 6};
+  constexpr int PB[4] = {3, 4, 7, 8};
   const T ics2 = K.ics2, ics4 = K.ics4;
-  const T u2 = ux * ux + uy * uy;
-  T feq[9], fb[9];
+  // moments of the Guo source first; the equilibrium part then starts from
+  // copies of rho and u that depend on them, so that the two sets of pair
+  // sums and differences are never live together
+  T mf[9];
+  if constexpr (forced) {
+    T fS[4], fD[4];
+    const T f0 = guo_k<T, FORCING>(0, ux, uy, fx, fy, K.w[0], ics2, ics4);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    T wk = K.w[k];
-    feq[k] = DEV ? deq_k<T>(k, drho, rho, ux, uy, u2, wk, ics2, ics4)
-                 : feq_k<T>(k, rho, ux, uy, u2, wk, ics2, ics4);
-    if constexpr (FORCING != D2Q9_FORCING_NONE)
-      fb[k] = guo_k<T, FORCING>(k, ux, uy, fx, fy, wk, ics2, ics4);
-    else
-      fb[k] = (T)0;
+    for (int n = 0; n < 4; ++n) {
+      const int k = PK[n], kb = PB[n];
+      const T fk = guo_k<T, FORCING>(k, ux, uy, fx, fy, K.w[k], ics2, ics4);
+      const T fkb = guo_k<T, FORCING>(kb, ux, uy, fx, fy, K.w[kb], ics2, ics4);
+      fS[n] = fk + fkb;
+      fD[n] = fk - fkb;
+    }
+    pair_moments<T>(f0, fS, fD, mf);
+    asm volatile("" : "+v"(rho), "+v"(ux), "+v"(uy)
+                 : "v"(mf[0]), "v"(mf[1]), "v"(mf[2]), "v"(mf[3]), "v"(mf[4]),
+                   "v"(mf[5]), "v"(mf[6]), "v"(mf[7]), "v"(mf[8]));
   }
-  T gm[9];
+  const T u2 = ux * ux + uy * uy;
+  const T e0 = g[0] - feq_k<T>(0, rho, ux, uy, u2, K.w[0], ics2, ics4);
+  T eS[4], eD[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const int k = PK[n], kb = PB[n];
+    const T ek = g[k] - feq_k<T>(k, rho, ux, uy, u2, K.w[k], ics2, ics4);
+    const T ekb = g[kb] - feq_k<T>(kb, rho, ux, uy, u2, K.w[kb], ics2, ics4);
+    eS[n] = ek + ekb;
+    eD[n] = ek - ekb;
+  }
+  T me[9], gm[9];
+  pair_moments<T>(e0, eS, eD, me);
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
-    T a = (T)0, b = (T)0, d = (T)0;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      if (MM[i][k] == 0) continue;
-      a = a + (T)MM[i][k] * g[k];
-      b = b + (T)MM[i][k] * feq[k];
-      if constexpr (FORCING != D2Q9_FORCING_NONE) d = d + (T)MM[i][k] * fb[k];
-    }
-    T s = K.rates[i];
-    T dm = s * (a - b) - ((T)1 - (T)0.5 * s) * d;
-    gm[i] = dm * (T)(1.0 / (double)ROWNORM[i]);
+    const T s = K.rates[i];
+    T p = ((T)1 - s) * me[i];
+    if constexpr (forced) p = p + ((T)1 - (T)0.5 * s) * mf[i];
+    gm[i] = p * (T)(1.0 / (double)ROWNORM[i]);
   }
+  T out[9];
+  pair_inverse<T>(gm, out);
+  T r2 = rho, x2 = ux, y2 = uy;
+  asm volatile("" : "+v"(r2), "+v"(x2), "+v"(y2));
+  const T u2b = x2 * x2 + y2 * y2;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    T acc = (T)0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-      if (MM[i][k] == 0) continue;
-      acc = acc + (T)MM[i][k] * gm[i];
-    }
-    g[k] = g[k] - acc;
-  }
+  for (int k = 0; k < 9; ++k)
+    g[k] = feq_k<T>(k, r2, x2, y2, u2b, K.w[k], ics2, ics4) + out[k];
+  (void)drho;
 }
 
 template <typename T, int MODEL, int FORCING, bool DEV>

@@ -13,6 +13,10 @@
 
 namespace d2q9 {
 
+// The argument checks of every single-lattice entry point (d2q9_step.hip):
+// 0, or D2Q9_E_ARG with the error text set.
+int validate_lattice(const d2q9_lattice* L);
+
 constexpr int RED_THREADS = 256;
 constexpr int RED_MAX_BLOCKS = 2048;
 

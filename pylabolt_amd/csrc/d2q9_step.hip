@@ -178,7 +178,7 @@ __device__ __forceinline__ void deep_wall_global(const d2q9_lattice& L,
 // Guo forcing needs 94 VGPRs and, held to 64, spilled 196 bytes per lane
 // and ran 3x slower.
 template <int S, int MODEL, int FORCING>
-__global__ __launch_bounds__(64 * DEEP_WAVES, Codec<S>::dev ? 8 : 4) void k_deep2(
+__global__ __launch_bounds__(64 * DEEP_WAVES, 8) void k_deep2(
     const d2q9_lattice L, const int32_t* __restrict__ deep, const uint8_t* __restrict__ walls,
     int n_deep, int ntj, const typename Codec<S>::raw* __restrict__ fsrc,
     typename Codec<S>::raw* __restrict__ fdst, typename Codec<S>::T* __restrict__ mdst,
@@ -595,6 +595,8 @@ int dispatch(const d2q9_lattice* L, const void* fsrc, void* fdst, const void* ms
 }
 
 }  // namespace
+
+int d2q9::validate_lattice(const d2q9_lattice* L) { return validate(L); }
 
 extern "C" {
 

@@ -29,6 +29,7 @@ reference's either way.
 """
 import ctypes
 import math
+import warnings
 from types import SimpleNamespace
 
 import numpy as np
@@ -527,8 +528,11 @@ class D2Q9Engine:
     ``[size, 2]``) that replaces ``rho * gravity``.  ``solid_tiles`` picks
     how tiles holding solids are swept (see ``CaseTables.set_tile``);
     ``two_step`` advances pairs of steps with one sweep over the deep
-    interior (d2q9_fused_step2) when deep tiles fit; by default when the
-    two population buffers exceed ``TWO_STEP_MIN_BYTES``.
+    interior (d2q9_fused_step2) when deep tiles fit; by default for FP64
+    and FP32 storage when the two population buffers exceed
+    ``TWO_STEP_MIN_BYTES`` (the results are bitwise those of single
+    steps), for 16-bit storage only when asked (its intermediate step stays
+    in FP32, which changes the rounding).
     """
 
     def __init__(self, nx, ny, solid_pad, density, velocity, elements,
@@ -643,7 +647,12 @@ class D2Q9Engine:
         # for lattices whose two population buffers exceed
         # TWO_STEP_MIN_BYTES
         if two_step is None:
-            two_step = (2 * self.f[0].numel() * self.f[0].element_size()
+            # FP64 and FP32 give the same bits either way; in 16-bit
+            # storage the two-step sweep keeps the intermediate step in FP32
+            # and so changes the numerics, which only an explicit request
+            # may do
+            two_step = (self.storage in (nt.STORE_F64, nt.STORE_F32) and
+                        2 * self.f[0].numel() * self.f[0].element_size()
                         > TWO_STEP_MIN_BYTES)
         self._two_step_req = bool(two_step)
         self.two_step = False
@@ -705,10 +714,18 @@ class D2Q9Engine:
             if not hasattr(self, "f_tmp"):
                 # initialize() copies f[0] into it: ghost and solid cells
                 # of the scratch lattice hold the encoded zero, like those
-                # of f[0] and f[1]
-                self.f_tmp = torch.empty_like(self.f[0])
-                self.mom_tmp = torch.zeros(3 * self.plane, dtype=self.ctype,
-                                           device=dev)
+                # of f[0] and f[1].  Without room for it the engine keeps
+                # single steps.
+                try:
+                    self.f_tmp = torch.empty_like(self.f[0])
+                    self.mom_tmp = torch.zeros(3 * self.plane,
+                                               dtype=self.ctype, device=dev)
+                except torch.cuda.OutOfMemoryError:
+                    self.__dict__.pop("f_tmp", None)
+                    self._deep = None
+                    warnings.warn("no device memory for the two-step sweep's "
+                                  "scratch lattice: single steps")
+                    return
         t = self._deep
         rims = []
         for fix, nfix in ((t["fix_a"], d.n_fix_a), (t["fix_b"], d.n_fix_b)):
@@ -873,9 +890,10 @@ class D2Q9Engine:
             n -= 1
         if self.two_step:
             # pairs of steps: graphs of graph_steps steps, the last pair
-            # eager so that it can store the moments
+            # eager when it has to store the moments
+            keep = 2 if write_moments else 0
             if use_graph:
-                while n - 2 >= self.graph_steps:
+                while n - keep >= self.graph_steps:
                     g = self._graph_for_state()
                     nt.check(self.lib.d2q9_graph_launch(g, self._stream_ptr),
                              "graph launch")
@@ -885,9 +903,10 @@ class D2Q9Engine:
                 self.step2(write_moments and n == 2)
                 n -= 2
         # full chunks of fused steps replay a captured graph; the last step
-        # stays eager so that it can store the moments
+        # stays eager when it has to store the moments
+        keep = 1 if write_moments else 0
         if use_graph:
-            while n - 1 >= self.graph_steps:
+            while n - keep >= self.graph_steps:
                 g = self._graph_for_state()
                 nt.check(self.lib.d2q9_graph_launch(g, self._stream_ptr),
                          "graph launch")
@@ -896,6 +915,15 @@ class D2Q9Engine:
         while n > 0:
             self.step(write_moments and n == 1)
             n -= 1
+
+    def capture_graph(self):
+        """Capture (once) the graph that ``advance`` replays from the
+        current buffer roles, without running it; a timed region that
+        follows then replays graphs only."""
+        if self.i_pop_new is None:
+            full = self.mom_full
+            self._graph_for_state()
+            self.mom_full = full
 
     def _graph_for_state(self):
         key = (self.i_pop, self.i_mom)

@@ -66,7 +66,10 @@ class Solver:
     how tiles holding solid cells are swept (``solid_tiles``: "sweep",
     "cells" or "auto"; None picks per storage form) and whether pairs of
     steps run as one sweep over the deep interior (``two_step``; None: on
-    when the two population buffers exceed 64 MiB).  Every choice gives bitwise the same results.
+    for FP64 and FP32 storage when the two population buffers exceed 64
+    MiB, off for 16-bit storage).  Every choice gives bitwise the same
+    results, but for ``two_step`` in 16-bit storage, whose intermediate
+    step stays in FP32.
     """
 
     def __init__(self, comm=None, backend="gpu", n_threads=None,
@@ -117,6 +120,10 @@ class Solver:
         """Build the device state (fluidLB.py:186-204)."""
         self.backend.create_engine(self.state, self.collision_operator,
                                    self.force_operator)
+        eng = self.backend.engine
+        print_log(f"{'Engine':<10}: storage {eng.storage_name}, two-step "
+                  f"sweep {'on' if eng.two_step else 'off'}",
+                  0, self.verbose if verbose is None else verbose)
         for op in (self.mpi_operator, self.collision_operator,
                    self.force_operator, self.compute_fields_operator,
                    self.streaming_operator, self.boundary_operator,
@@ -181,6 +188,12 @@ class Solver:
                 res_velocity=self.residue_operator.residues["res_velocity"])
             self.io_operator.write_fields(st, self.backend, t, solver=self)
             self.io_operator.write_histories(st, t)
+        if end > st.control.start_time and not self._obstacle_event(end):
+            # the reference leaves the last step's obstacle forces in place
+            # (it evaluates them every step); the loop above evaluated them
+            # at the write steps only
+            self.obstacle_operator.compute_force_torque(st, self.backend,
+                                                        self.mpi_operator)
         self.engine.synchronize()
         self.run_time = time.perf_counter() - t0
         print_log(f"{'Simulation complete, run time':<30}: "

@@ -47,13 +47,18 @@ class InputOutputOperator:
                               "ny": int(state.domain.no_of_procs_y)},
             "fields_saved": self.fields_save_metadata,
         }
+        eng = getattr(self, "engine", None)
+        if eng is not None:
+            # not read by the reference's tools: how the run was computed
+            meta["engine"] = {"storage": eng.storage_name,
+                              "two_step": bool(eng.two_step)}
         os.makedirs(self.field_save_path, exist_ok=True)
         with open(os.path.join(self.root, "metadata.json"), "w") as f:
             json.dump(meta, f, indent=4)
         self._dumped = True
 
     def set_backend(self, state, backend, verbose=True):
-        return None
+        self.engine = backend.engine
 
     def compile(self, state, backend, verbose=True):
         return None

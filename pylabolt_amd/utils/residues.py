@@ -19,10 +19,12 @@ class ResidueOperator:
         self.mom_old = None
 
     def set_backend(self, state, backend, verbose=True):
+        # phi_old starts at zero (residues.py:53-64 allocates it with
+        # np.zeros_like and set_backend copies it to the device), so the
+        # first residue compares against zero fields
         eng = backend.engine
-        self.mom_old = torch.zeros_like(eng.mom[0])
-        # phi_old starts as the initial fields (residues.py:53-64)
-        self.mom_old.copy_(eng.mom[eng.i_mom])
+        with torch.cuda.stream(eng.stream):
+            self.mom_old = torch.zeros_like(eng.mom[0])
 
     def compile(self, state, backend, verbose=True):
         return None

@@ -1,0 +1,153 @@
+"""The workloads at their real sizes and lengths.
+
+* Lid-driven cavity 128 x 128, Re = 100, 10,000 steps (the full workload):
+  the FP64 engine against the C++ restatement (bar 1e-12, observed
+  bitwise), then the engine's own steady centre line against Ghia, Ghia &
+  Shin (1982).
+* Gravity-driven channel 4096 x 1024, MRT + second-order Guo, FP64: against
+  the C++ restatement after 100 steps (bar 1e-12), and at its full 20,000
+  steps against the start-up series of plane Poiseuille flow (halfway
+  bounce-back walls at y = 0 and H, nu t / H^2 = 0.0098): the profile is
+  still a transient, so the series, not the steady parabola, is the
+  reference.
+* The 256-lattice batch (seed 0, 44.3 M cells, half cavities BGK, half MRT
+  + Guo channels): FP32, every lattice's populations and moments
+  bitwise equal to the same lattice advanced on its own (stored words of
+  every fluid cell, and the moments).
+"""
+import numpy as np
+import pytest
+import torch
+
+from pylabolt_amd import cases
+from pylabolt_amd.engine import pop_aos
+from tests.helpers import max_rel, oracle_case
+from tests.test_oracle import GHIA_U, GHIA_Y
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(sim, storage="f64", **kw):
+    from pylabolt_amd.solvers.fluidLB import Solver
+    s = Solver(simulation=sim, storage=storage, verbose=False, **kw)
+    s.set_backend()
+    return s
+
+
+def _compare(eng, case, label, tol=1e-12):
+    fl = case.fluid
+    rho, vel = eng.moments_host()
+    errs = {"pop": max_rel(eng.pop()[fl], case.pop[fl]),
+            "pop_new": max_rel(eng.pop_new()[fl], case.pop_new[fl]),
+            "density": max_rel(rho[fl], case.density[fl]),
+            "velocity": max_rel(vel[fl], case.velocity[fl])}
+    exact = all(v == 0.0 for v in errs.values())
+    print(f"[config] {label}: {errs} bitwise={exact}", flush=True)
+    for k, v in errs.items():
+        assert v <= tol, f"{label}: {k} max rel err {v}"
+
+
+def test_cavity_128_full_length_vs_cpp_then_ghia(ref_lib):
+    n = 128
+    sim = cases.lid_driven_cavity(n, n)
+    case = oracle_case(sim)
+    r = ref_lib.RefRunner(case)
+    r.initialize()
+    s = _solver(sim)
+    s.advance(10000)
+    r.run(10000)
+    _compare(s.engine, case, "cavity 128x128 10,000 steps")
+    # the engine's own flow, run on to steady state (40 lid transits)
+    s.advance(41200)
+    rho, vel = s.engine.moments_host()
+    u = vel.reshape(n + 2, n + 2, 2)
+    uc = 0.5 * (u[n // 2, 1:-1, 0] + u[n // 2 + 1, 1:-1, 0]) / 0.1
+    y = (np.arange(1, n + 1) - 0.5) / n
+    prof = np.interp(GHIA_Y, np.concatenate([[0], y, [1]]),
+                     np.concatenate([[0], uc, [1]]))
+    err = np.max(np.abs(prof - GHIA_U))
+    print(f"[config] cavity 128x128 Re=100 max |u - Ghia| = {err:.4f}",
+          flush=True)
+    assert err <= 0.02
+
+
+def _channel():
+    return cases.poiseuille_channel(4096, 1024, model="MRT",
+                                    forcing="guo_second_order")
+
+
+def test_channel_4096x1024_mrt_guo_vs_cpp(ref_lib):
+    sim = _channel()
+    case = oracle_case(sim)
+    r = ref_lib.RefRunner(case)
+    r.initialize()
+    s = _solver(sim)
+    s.advance(100)
+    r.run(100)
+    _compare(s.engine, case, "channel 4096x1024 MRT+Guo 100 steps")
+
+
+def _series(y, H, nu, g, t, terms=400):
+    u_max = g * H * H / (8 * nu)
+    u = 4 * u_max * (y / H) * (1 - y / H)
+    for k in range(terms):
+        m = 2 * k + 1
+        u -= (32 * u_max / (np.pi ** 3 * m ** 3)) * np.sin(m * np.pi * y / H) \
+            * np.exp(-(m * np.pi / H) ** 2 * nu * t)
+    return u
+
+
+def test_channel_4096x1024_mrt_guo_startup_series():
+    sim = _channel()
+    nu = sim.transport_dict["kin_visc"]
+    g = sim.forcing_dict["gravity"][0]
+    H = 1024
+    s = _solver(sim)
+    steps = sim.control_dict["end_time"]
+    s.advance(steps)
+    rho, vel = s.engine.moments_host()
+    u = vel.reshape(4096 + 2, H + 2, 2)[1:-1, 1:-1, 0]
+    prof = u.mean(axis=0)
+    spread = float(np.abs(u - prof).max())
+    y = np.arange(1, H + 1) - 0.5
+    ref = _series(y, H, nu, g, steps)
+    rel = float(np.linalg.norm(prof - ref) / np.linalg.norm(ref))
+    print(f"[config] channel 4096x1024 MRT+Guo {steps} steps: rel L2 vs "
+          f"start-up series {rel:.2e}, centre {prof[H // 2]:.6e} vs "
+          f"{ref[H // 2]:.6e}, x-spread {spread:.1e}", flush=True)
+    assert rel <= 1e-4
+    assert spread <= 1e-12 * float(np.abs(prof).max())
+
+
+def test_batched_256_lattices_fp32_bitwise_per_lattice():
+    from pylabolt_amd.batch import BatchConfig, BatchedLattices
+    sims = cases.batch_cases()
+    cfg = BatchConfig.from_dict({"tile_i": 1, "policy": "static",
+                                 "grid": 4096, "chunk": 1, "waves": 0})
+    batch = BatchedLattices(sims, storage="f32", config=cfg, graph_steps=8)
+    cells = sum(e.nx * e.ny for e in batch.engines)
+    assert len(batch.engines) == 256 and cells == 44275712
+    solo = [_solver(s, "f32").engine for s in cases.batch_cases()]
+    for n in (3, 16):
+        batch.advance(n)
+        for e in solo:
+            e.advance(n)
+    torch.cuda.synchronize()
+
+    def fluid_pops(e, idx):
+        # the stored words of the fluid cells, compared on the device
+        aos = pop_aos(e.f[idx], e.nx, e.ny, e.B, e.nbj)
+        keep = torch.from_numpy(e.tables.fluid_pad.ravel()).to(aos.device)
+        return aos[keep]
+
+    for l, (a, b) in enumerate(zip(batch.engines, solo)):
+        assert torch.equal(fluid_pops(a, a.i_pop), fluid_pops(b, b.i_pop)), \
+            f"lattice {l}: pop"
+        assert torch.equal(a.mom[a.i_mom], b.mom[b.i_mom]), \
+            f"lattice {l}: moments"
+        assert torch.equal(fluid_pops(a, a.materialize()),
+                           fluid_pops(b, b.materialize())), \
+            f"lattice {l}: pop_new"
+    print(f"[config] 256 lattices ({cells} cells) FP32 batch: 19 steps "
+          f"bitwise equal to per-lattice", flush=True)
+    batch.close()

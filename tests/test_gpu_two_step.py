@@ -189,18 +189,23 @@ def test_two_step_force_field_bitwise():
 
 
 def test_two_step_default_threshold():
-    """By default the two-step sweep runs when the two population buffers
-    exceed TWO_STEP_MIN_BYTES: not on the 1024x256 FP64 channel (54 MiB),
-    on the 2048x1024 FP16 one (90 MiB)."""
+    """By default the two-step sweep runs in FP64 and FP32 storage when the
+    two population buffers exceed TWO_STEP_MIN_BYTES: not on the 1024x256
+    FP64 channel (54 MiB), on the 2048x512 FP64 (180 MiB) and 2048x1024
+    FP32 (180 MiB) ones.  In 16-bit storage it changes the rounding, so it
+    runs only when asked, whatever the size (2048x1024 FP16, 90 MiB)."""
     from pylabolt_amd.engine import TWO_STEP_MIN_BYTES
     from pylabolt_amd.solvers.fluidLB import Solver
-    for nx, ny, storage, want in ((1024, 256, "f64", False),
-                                  (2048, 1024, "f16dev", True)):
+    for nx, ny, storage, big, want in (
+            (1024, 256, "f64", False, False),
+            (2048, 512, "f64", True, True),
+            (2048, 1024, "f32", True, True),
+            (2048, 1024, "f16dev", True, False)):
         s = Solver(simulation=cases.poiseuille_channel(nx, ny),
                    storage=storage, verbose=False)
         s.set_backend()
         e = s.engine
         nbytes = 2 * e.f[0].numel() * e.f[0].element_size()
-        assert (nbytes > TWO_STEP_MIN_BYTES) == want
+        assert (nbytes > TWO_STEP_MIN_BYTES) == big
         assert e.two_step == want, (nx, ny, storage, nbytes)
         s.advance(6)

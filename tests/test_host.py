@@ -199,6 +199,41 @@ def test_circle_rasterisation_reference_fixture(center):
     assert np.all(st.fields.solid_id[~solid] == -1)
 
 
+def _two_circles_periodic():
+    sim = cases.poiseuille_channel(30, 22, model="BGK", forcing=None)
+    sim.obstacle_dict = {
+        "options": {},
+        "edge": {"type": "circle", "radius": 4.0, "center": [1.0, 10.3],
+                 "density": 1.0, "static": True},
+        "mid": {"type": "circle", "radius": 3.5, "center": [16.2, 11.0],
+                "density": 1.0, "static": True}}
+    return sim
+
+
+@pytest.mark.parametrize("make", [
+    lambda: cases.cylinder(40, 24, diameter=8, xc=12),
+    _two_circles_periodic])
+def test_obstacle_boundary_fields_match_oracle(make):
+    """solid_boundary, fluid_boundary, solid_id and the circle normals the
+    obstacle operator marks at set-up (saved in every t_<step>.npz) equal
+    the oracle's restatement of compute_obstacle_boundary and
+    compute_normals_circle, here also across a periodic edge."""
+    from oracle import d2q9_oracle as O
+    from pylabolt_amd.base.operators import mark_obstacle_boundaries
+    st = State(make())
+    f = st.fields
+    circles = [(o.id, o.center) for o in st.obstacle.obstacles]
+    want = O.obstacle_boundary_fields(
+        tuple(int(v) for v in st.domain.shape), f.solid, f.ghost_node,
+        f.solid_id, circles, st.boundary.x_periodic, st.boundary.y_periodic)
+    mark_obstacle_boundaries(st)
+    assert f.solid_boundary.any() and f.fluid_boundary.any()
+    assert np.array_equal(f.solid_boundary, want[0])
+    assert np.array_equal(f.fluid_boundary, want[1])
+    assert np.array_equal(f.solid_id, want[2])
+    assert np.array_equal(f.surface_normals, want[3])
+
+
 def test_single_rank_comm_abort_raises():
     with pytest.raises(SystemExit):
         SingleRankComm().Abort()
