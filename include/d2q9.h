@@ -255,6 +255,18 @@ int64_t d2q9_reduce_work_len(const d2q9_lattice* L);
 int d2q9_mass_fp64(const d2q9_lattice* L, const void* f, double* work,
                    double* out, void* stream);
 
+/* d2q9_mass_fp64 with no workspace argument and no allocation.  It has no
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  The
+ * partial sums live in a static device array of 2048 doubles (16 KiB per
+ * GPU, held until the process ends), which bounds nothing about the
+ * lattice (a reduction launches at most 2048 workgroups); a launch needing
+ * more returns D2Q9_E_TOO_LARGE.  Calls that overlap on one GPU share that
+ * array and race: order them on one stream.  Same kernels, same result,
+ * bit for bit, as d2q9_mass_fp64. */
+int d2q9_mass_fp64_static(const d2q9_lattice* L, const void* f, double* out,
+                          void* stream);
+
 /* Residue sums over fluid cells of mom vs mom_old (FP64):
  * out[0..5] = sum drho^2, sum rho_old^2, sum dux^2, sum ux_old^2,
  *             sum duy^2, sum uy_old^2; then mom_old = mom. */

@@ -114,6 +114,7 @@ SIGNATURES = {
     "d2q9_moments": (ctypes.c_int, [_LP, _P, _P, _P]),
     "d2q9_reduce_work_len": (ctypes.c_int64, [_LP]),
     "d2q9_mass_fp64": (ctypes.c_int, [_LP, _P, _P, _P, _P]),
+    "d2q9_mass_fp64_static": (ctypes.c_int, [_LP, _P, _P, _P]),
     "d2q9_residues": (ctypes.c_int, [_LP, _P, _P, _P, _P, _P]),
     "d2q9_copy": (ctypes.c_int, [_P, _P, ctypes.c_int64, _P]),
     "d2q9_graph_begin": (ctypes.c_int, [_P]),
