@@ -104,6 +104,12 @@ def build(verbose=True, extra=(), reuse_objects=False):
         res = list(pool.map(
             lambda s: _compile(s, extra, verbose, digest, obj_dir), srcs))
     objs = [o for o, _ in res]
+    if obj_dir is not None:
+        # keep only the objects of the current sources
+        keep = {os.path.basename(o) for o in objs}
+        for f in os.listdir(obj_dir):
+            if f.endswith(".o") and f not in keep:
+                os.remove(os.path.join(obj_dir, f))
     tmp = LIB + ".tmp"
     subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC",
                     *objs, "-o", tmp], check=True)
