@@ -294,10 +294,24 @@ __device__ __forceinline__ int64_t pop_index(const d2q9_lattice& L, int i, int j
 // Equilibrium and Guo source.
 // ---------------------------------------------------------------------------
 // Reference form: feq_k = (w rho) (((1 + ics2 cu) + ((0.5 ics4) cu) cu) - (0.5 ics2) u2)
+// c_k . u with the zero components of c_k left out and the unit ones as a
+// sign: for finite u the same bits as cx*ux + cy*uy (a product by +-1 is
+// exact, and a +-0 term leaves a finite sum unchanged but for the sign of
+// a zero, which only ever enters through 1 + ics2 cu and cu*cu), without
+// the products by zero IEEE arithmetic would keep.
+template <typename T>
+__device__ __forceinline__ T cdot(int k, T ux, T uy) {
+  if (CX[k] == 0 && CY[k] == 0) return (T)0;
+  if (CX[k] == 0) return CY[k] > 0 ? uy : -uy;
+  const T a = CX[k] > 0 ? ux : -ux;
+  if (CY[k] == 0) return a;
+  return CY[k] > 0 ? a + uy : a - uy;
+}
+
 template <typename T>
 __device__ __forceinline__ T feq_k(int k, T rho, T ux, T uy, T u2, T wk,
                                    T ics2, T ics4) {
-  T cu = (T)CX[k] * ux + (T)CY[k] * uy;
+  T cu = cdot<T>(k, ux, uy);
   return (wk * rho) *
          (((T)1 + ics2 * cu + ((T)0.5 * ics4) * cu * cu) - ((T)0.5 * ics2) * u2);
 }
@@ -306,7 +320,7 @@ __device__ __forceinline__ T feq_k(int k, T rho, T ux, T uy, T u2, T wk,
 template <typename T>
 __device__ __forceinline__ T deq_k(int k, T drho, T rho, T ux, T uy, T u2, T wk,
                                    T ics2, T ics4) {
-  T cu = (T)CX[k] * ux + (T)CY[k] * uy;
+  T cu = cdot<T>(k, ux, uy);
   return wk * (drho + rho * (ics2 * cu + ((T)0.5 * ics4) * cu * cu - ((T)0.5 * ics2) * u2));
 }
 
@@ -314,12 +328,16 @@ __device__ __forceinline__ T deq_k(int k, T drho, T rho, T ux, T uy, T u2, T wk,
 template <typename T, int FORCING>
 __device__ __forceinline__ T guo_k(int k, T ux, T uy, T fx, T fy, T wk,
                                    T ics2, T ics4) {
+  // zero components left out as in cdot: c - u is -u for c = 0, and the
+  // (cu c) ics4 term adds a zero
   if constexpr (FORCING == D2Q9_GUO_LINEAR) {
-    return (wk * ((T)CX[k] * fx + (T)CY[k] * fy)) * ics2;
+    return (wk * cdot<T>(k, fx, fy)) * ics2;
   } else {
-    T cu = (T)CX[k] * ux + (T)CY[k] * uy;
-    T const_x = ((T)CX[k] - ux) * ics2 + (cu * (T)CX[k]) * ics4;
-    T const_y = ((T)CY[k] - uy) * ics2 + (cu * (T)CY[k]) * ics4;
+    T cu = cdot<T>(k, ux, uy);
+    T const_x = ((T)CX[k] - ux) * ics2;
+    if (CX[k] != 0) const_x = const_x + (CX[k] > 0 ? cu : -cu) * ics4;
+    T const_y = ((T)CY[k] - uy) * ics2;
+    if (CY[k] != 0) const_y = const_y + (CY[k] > 0 ? cu : -cu) * ics4;
     return wk * (const_x * fx + const_y * fy);
   }
 }
@@ -387,11 +405,13 @@ __device__ __forceinline__ void moments(const T (&g)[9], const d2q9_lattice& L,
     fx = rho * K.gx;
     fy = rho * K.gy;
   }
+  // sum c_k g_k from 0 in k order; zero terms left out (a partial sum that
+  // starts at +0 is never -0, so adding +-0 would leave it unchanged)
   T sx = (T)0, sy = (T)0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    sx += (T)CX[k] * g[k];
-    sy += (T)CY[k] * g[k];
+    if (CX[k] != 0) sx = CX[k] > 0 ? sx + g[k] : sx - g[k];
+    if (CY[k] != 0) sy = CY[k] > 0 ? sy + g[k] : sy - g[k];
   }
   // FP64 divides as the reference does; the float forms use the
   // hardware reciprocal (1 ulp)
@@ -411,12 +431,14 @@ __device__ __forceinline__ void moments(const T (&g)[9], const d2q9_lattice& L,
   }
 }
 
-// Float BGK (deviation form) over the four opposite-direction pairs: with
-// cu_kb = -cu_k, the equilibrium deviation splits into a part shared by
-// the pair, w (drho + rho (0.5 ics4 cu^2 - 0.5 ics2 u^2)), and one of
-// opposite sign, w rho ics2 cu; the Guo source likewise into
-// w (ics4 cu (c.F) - ics2 (u.F)) and w ics2 (c.F) (second order) or 0 and
-// w ics2 (c.F) (linear).  g <- g + om (geq - g) (+ (1 - om/2) Fbar).
+// Float BGK (deviation form) over the four opposite-direction pairs.  With
+// cu_kb = -cu_k the relaxed equilibrium om geq splits into a part shared by
+// the pair, A = om w (drho + rho (0.5 ics4 cu^2 - 0.5 ics2 u^2)), and one of
+// opposite sign, B = om w rho ics2 cu; the Guo source (times 1 - om/2)
+// likewise into (1 - om/2) w (ics4 cu (c.F) - ics2 (u.F)) (second order
+// only) and (1 - om/2) w ics2 (c.F).  Then g_k <- (1 - om) g_k + (A + B),
+// g_kb <- (1 - om) g_kb + (A - B), with om w rho and om w drho formed once
+// per cell and weight.
 template <int FORCING>
 __device__ __forceinline__ void collide_bgk_pairs(float (&g)[9], const Consts<float>& K,
                                                   float drho, float rho, float ux, float uy,
@@ -424,42 +446,47 @@ __device__ __forceinline__ void collide_bgk_pairs(float (&g)[9], const Consts<fl
   // fused multiply-adds are written out (never left to the compiler), so
   // every kernel build computes the same values
   const float om = K.om;
+  const float omc = 1.0f - om;
   const float ics2 = K.ics2;
   const float hics4 = 0.5f * K.ics4;
   const float hu2 = (0.5f * ics2) * __builtin_fmaf(ux, ux, uy * uy);
-  const float rics2 = rho * ics2;
   const float cf = __builtin_fmaf(-0.5f, om, 1.0f);
-  const float uf = ics2 * __builtin_fmaf(ux, fx, uy * fy);
+  float uf = 0.0f;
+  if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) uf = ics2 * __builtin_fmaf(ux, fx, uy * fy);
   // rest direction (cu = 0, c = 0)
   {
-    const float eq = K.w[0] * __builtin_fmaf(-rho, hu2, drho);
-    float v = __builtin_fmaf(om, eq - g[0], g[0]);
-    if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) v = __builtin_fmaf(cf, -(K.w[0] * uf), v);
-    g[0] = v;
+    const float omw = om * K.w[0];
+    float a = __builtin_fmaf(-(rho * omw), hu2, drho * omw);
+    if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) a = __builtin_fmaf(cf * K.w[0], -uf, a);
+    g[0] = __builtin_fmaf(omc, g[0], a);
   }
   constexpr int PK[4] = {1, 2, 5, 6};
   constexpr int PB[4] = {3, 4, 7, 8};
+  // per weight class (axis pairs 0, 1; diagonal pairs 2, 3)
+  float rw[2], dw[2], aw[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const float omw = om * K.w[c ? 5 : 1];
+    rw[c] = rho * omw;
+    dw[c] = drho * omw;
+    aw[c] = rw[c] * ics2;
+  }
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    const int k = PK[n], kb = PB[n];
+    const int k = PK[n], kb = PB[n], c = n >> 1;
     const float cu = (float)CX[k] * ux + (float)CY[k] * uy;
-    const float w = K.w[k];
-    const float sym = __builtin_fmaf(rho, __builtin_fmaf(hics4 * cu, cu, -hu2), drho);
-    const float asym = rics2 * cu;
-    const float eqk = w * (sym + asym), eqb = w * (sym - asym);
-    float vk = __builtin_fmaf(om, eqk - g[k], g[k]);
-    float vb = __builtin_fmaf(om, eqb - g[kb], g[kb]);
+    const float t = __builtin_fmaf(hics4 * cu, cu, -hu2);
+    float A = __builtin_fmaf(rw[c], t, dw[c]);
+    float B = aw[c] * cu;
     if constexpr (FORCING != D2Q9_FORCING_NONE) {
+      const float w = K.w[k];
       const float cF = (float)CX[k] * fx + (float)CY[k] * fy;
-      const float fa = (w * ics2) * cF;
-      float fs = 0.0f;
+      B = __builtin_fmaf((cf * w) * ics2, cF, B);
       if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER)
-        fs = w * __builtin_fmaf(K.ics4 * cu, cF, -uf);
-      vk = __builtin_fmaf(cf, fs + fa, vk);
-      vb = __builtin_fmaf(cf, fs - fa, vb);
+        A = __builtin_fmaf(cf * w, __builtin_fmaf(K.ics4 * cu, cF, -uf), A);
     }
-    g[k] = vk;
-    g[kb] = vb;
+    g[k] = __builtin_fmaf(omc, g[k], A + B);
+    g[kb] = __builtin_fmaf(omc, g[kb], A - B);
   }
 }
 

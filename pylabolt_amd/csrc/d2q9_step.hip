@@ -128,7 +128,15 @@ __global__ __launch_bounds__(256) void k_step_all(
 // and FP32 storage the result is bitwise that of two single steps; in the
 // 16-bit forms the intermediate step stays in FP32 (LDS) instead of being
 // rounded to storage, and only the stored step is rounded.
-constexpr int DEEP_ROWS = 12;
+// Build options (development variants): the tile height and the waves per
+// SIMD the deep kernel is held to.
+#ifndef D2Q9_DEEP_ROWS
+#define D2Q9_DEEP_ROWS 12
+#endif
+#ifndef D2Q9_DEEP_WAVES_PER_SIMD
+#define D2Q9_DEEP_WAVES_PER_SIMD 8
+#endif
+constexpr int DEEP_ROWS = D2Q9_DEEP_ROWS;
 constexpr int DEEP_R1 = DEEP_ROWS + 2;
 constexpr int DEEP_WAVES = DEEP_R1 + 2;
 
@@ -178,7 +186,7 @@ __device__ __forceinline__ void deep_wall_global(const d2q9_lattice& L,
 // Guo forcing needs 94 VGPRs and, held to 64, spilled 196 bytes per lane
 // and ran 3x slower.
 template <int S, int MODEL, int FORCING>
-__global__ __launch_bounds__(64 * DEEP_WAVES, 8) void k_deep2(
+__global__ __launch_bounds__(64 * DEEP_WAVES, D2Q9_DEEP_WAVES_PER_SIMD) void k_deep2(
     const d2q9_lattice L, const int32_t* __restrict__ deep, const uint8_t* __restrict__ walls,
     int n_deep, int ntj, const typename Codec<S>::raw* __restrict__ fsrc,
     typename Codec<S>::raw* __restrict__ fdst, typename Codec<S>::T* __restrict__ mdst,
