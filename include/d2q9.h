@@ -133,7 +133,11 @@ typedef struct d2q9_lattice {
   double w[9];
   double gravity[2];
   double rates[9];              /* MRT relaxation rates */
-  double dev_scale[9];          /* 16-bit storage scale per direction */
+  double dev_scale[9];          /* 16-bit storage scale per direction; a
+                                 * power of two keeps decoding exact
+                                 * (cf.inv_scale = 1 / scale), else every
+                                 * decoded value carries the same relative
+                                 * rounding error of inv_scale */
   d2q9_fconst cf;
   const uint8_t* flags;         /* [plane] */
   /* Tile lists: tiles[0 .. n_bulk) are bulk tiles (no solid cell; cells

@@ -31,9 +31,12 @@ engine identically (parity for them is against this definition):
   reference's moment matrix M and ``S = (1,1,1,1,1,1,1,omega,omega)``
   (base/collision_operator.py:136-165, whose own setup is broken); Fbar is
   the Guo source of the selected forcing without its (1 - omega/2) factor.
-  It is evaluated as ``feq + M^-1 [(I - S) M (f - feq) + (I - S/2) M Fbar]``
-  over the opposite-direction pairs; the operation order is spelled out in
-  ``mrt_collide``, ``pair_moments`` and ``pair_inverse``.
+  With unit rates on rows 0..6 it reduces to ``feq + 0.5 Fbar`` plus the
+  relaxed stress moments (p_xx, p_xy) of ``f - feq + 0.5 Fbar``; that
+  reduced form, in the operation order ``mrt_collide`` spells out, is the
+  definition the engine follows.  ``mrt_collide_general`` evaluates the
+  operator through the full moment map (``pair_moments``,
+  ``pair_inverse``) for any rates; the tests pin the reduced form to it.
 * zero_gradient: the unknown directions of a boundary node take the values
   the streaming pass produced in the same directions at the inward
   neighbour (before any boundary element ran).
@@ -41,6 +44,16 @@ engine identically (parity for them is against this definition):
   of ``rho * gravity``.
 * fixed_pressure uses the integer inward normal (the reference indexes with
   the float normal, fluid_boundary_kernels.py:133-135).
+
+Pins (tests/test_oracle.py, tests/test_oracle_pins.py): every function
+here is checked against the reference's own fixtures, closed forms
+(equilibrium and Guo-source moments, fixed points, exact shifts),
+analytic flows (start-up Poiseuille series for BGK and MRT, plane
+Poiseuille between pressure ends, Ghia's cavity) or an identity (MRT
+reduced form = full moment map; zero-gradient = periodic for fields
+independent of y), each with a planted mistake that fails.  The one
+assumption no check here can pin is Numba's rounding: one IEEE rounding
+per operation in source order, as without fastmath.
 
 The reference's CPU single-rank halo exchange sends and receives with its
 own rank (MPI_operator.py:261-369), which copies column 1 into ghost column
@@ -259,8 +272,8 @@ def pair_inverse(gm):
             sym68 - asym68]
 
 
-def mrt_collide(case, rates=None):
-    """Build-defined MRT with Guo forcing (see module docstring), in the
+def mrt_collide_general(case, rates=None):
+    """The MRT operator with any rates, through the full moment map, in the
     opposite-pair form:
 
       e_k = f_k - feq_k,  Fbar_k = Guo source without (1 - omega/2)
@@ -270,7 +283,10 @@ def mrt_collide(case, rates=None):
       f*_k = feq_k + (M^-1 p)_k     (pair_inverse of p_i * (1/|row_i|^2))
 
     which is f - M^-1 [S (M f - M feq) - (I - S/2) M Fbar] in exact
-    arithmetic; ``s = rates`` (default (1,1,1,1,1,1,1,omega,omega))."""
+    arithmetic; ``s = rates`` (default (1,1,1,1,1,1,1,omega,omega)).  The
+    time step uses ``mrt_collide``, the same operator for the reference's
+    rates in fewer operations; this form is the one the pins compare it
+    with (and the one that can take other rates)."""
     m_ = case.fluid
     c = case.const
     om = case.omega
@@ -308,6 +324,53 @@ def mrt_collide(case, rates=None):
     res = np.empty_like(pn)
     for k in range(9):
         res[:, k] = feq[k] + out[k]
+    case.pop[m_] = res
+
+
+def mrt_collide(case):
+    """Build-defined MRT with Guo forcing and the reference's rates
+    S = (1,1,1,1,1,1,1,omega,omega) (base/collision_operator.py:158-162).
+
+    With unit rates on rows 0..6 only the stress rows 7 (p_xx) and 8 (p_xy)
+    keep a part of f - feq, and M^-1 = M^T diag(1/|row|^2) with
+    |row 7|^2 = |row 8|^2 = 4, so the operator of ``mrt_collide_general``
+    becomes, in this operation order:
+
+      q_k = feq_k + 0.5 Fbar_k            (feq_k alone without forcing)
+      e_k = f_k - feq_k
+      a7 = (e1 + e3) - (e2 + e4) + 0.5 ((F1 + F3) - (F2 + F4))
+      a8 = (e5 + e7) - (e6 + e8) + 0.5 ((F5 + F7) - (F6 + F8))
+      r = ((1 - omega) * 0.25) * a
+      f*_0 = q_0;  f*_k = q_k + r7 (k = 1, 3), - r7 (2, 4),
+                          + r8 (5, 7), - r8 (6, 8)
+
+    (the signs are the entries of rows 7 and 8 of M)."""
+    m_ = case.fluid
+    c = case.const
+    om = case.omega
+    rho = case.density[m_]
+    ux, uy = case.velocity[m_, 0], case.velocity[m_, 1]
+    fx, fy = case.force_field[m_, 0], case.force_field[m_, 1]
+    pn = case.pop_new[m_]
+    q = [equilibrium(rho, ux, uy, k, c) for k in range(9)]
+    e = [pn[:, k] - q[k] for k in range(9)]
+    a7 = (e[1] + e[3]) - (e[2] + e[4])
+    a8 = (e[5] + e[7]) - (e[6] + e[8])
+    if case.forcing is not None:
+        fb = [guo_source(k, ux, uy, fx, fy, case.forcing, c)
+              for k in range(9)]
+        a7 = a7 + 0.5 * ((fb[1] + fb[3]) - (fb[2] + fb[4]))
+        a8 = a8 + 0.5 * ((fb[5] + fb[7]) - (fb[6] + fb[8]))
+        q = [q[k] + 0.5 * fb[k] for k in range(9)]
+    cs = (1.0 - om) * 0.25
+    r7 = cs * a7
+    r8 = cs * a8
+    res = np.empty_like(pn)
+    res[:, 0] = q[0]
+    for k, r in ((1, r7), (3, r7), (5, r8), (7, r8)):
+        res[:, k] = q[k] + r
+    for k, r in ((2, r7), (4, r7), (6, r8), (8, r8)):
+        res[:, k] = q[k] - r
     case.pop[m_] = res
 
 

@@ -19,9 +19,6 @@ namespace {
 const int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 const int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 const int INV[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
-// squared row norms of the MRT moment matrix (base/collision_operator.py
-// :147-157), whose rows are orthogonal: M^-1 = M^T diag(1 / ROWNORM)
-const int ROWNORM[9] = {9, 36, 36, 6, 12, 6, 12, 4, 4};
 }  // namespace
 
 extern "C" {
@@ -80,77 +77,38 @@ static inline double guo_k(const oracle_case* c, int k, double ux, double uy,
   return c->w[k] * (const_x * fx + const_y * fy);
 }
 
-// The nine moments (rows of M) of a population-like vector from its rest
-// value x0 and the sums S and differences D of the pairs (1,3), (2,4),
-// (5,7), (6,8); operation order of d2q9_oracle.pair_moments.
-static void pair_moments(double x0, const double* S, const double* D, double* m) {
-  const double s12 = S[0] + S[1], s34 = S[2] + S[3];
-  const double a = D[2] - D[3], b = D[2] + D[3];
-  m[0] = (x0 + s12) + s34;
-  m[1] = ((-4.0 * x0) - s12) + 2.0 * s34;
-  m[2] = ((4.0 * x0) + s34) - 2.0 * s12;
-  m[3] = D[0] + a;
-  m[4] = a - 2.0 * D[0];
-  m[5] = D[1] + b;
-  m[6] = b - 2.0 * D[1];
-  m[7] = S[0] - S[1];
-  m[8] = S[2] - S[3];
-}
-
-// M^T gm by columns; operation order of d2q9_oracle.pair_inverse.
-static void pair_inverse(const double* gm, double* out) {
-  out[0] = gm[0] + 4.0 * (gm[2] - gm[1]);
-  const double base13 = (gm[0] - gm[1]) - 2.0 * gm[2];
-  const double sym13 = base13 + gm[7], sym24 = base13 - gm[7];
-  const double asym13 = gm[3] - 2.0 * gm[4], asym24 = gm[5] - 2.0 * gm[6];
-  const double base57 = (gm[0] + gm[2]) + 2.0 * gm[1];
-  const double sym57 = base57 + gm[8], sym68 = base57 - gm[8];
-  const double p34 = gm[3] + gm[4], p56 = gm[5] + gm[6];
-  const double asym57 = p34 + p56, asym68 = p56 - p34;
-  out[1] = sym13 + asym13;
-  out[2] = sym24 + asym24;
-  out[3] = sym13 - asym13;
-  out[4] = sym24 - asym24;
-  out[5] = sym57 + asym57;
-  out[6] = sym68 + asym68;
-  out[7] = sym57 - asym57;
-  out[8] = sym68 - asym68;
-}
-
-// Build-defined MRT with Guo forcing, opposite-pair form of
-// d2q9_oracle.mrt_collide: f* = feq + M^-1 [(I - S) M (f - feq) +
-// (I - S/2) M Fbar], S = (1,1,1,1,1,1,1,omega,omega).
+// Build-defined MRT with Guo forcing and S = (1,1,1,1,1,1,1,omega,omega),
+// reduced form of d2q9_oracle.mrt_collide, same operation order:
+// q = feq (+ 0.5 Fbar), stress moments a7, a8 of f - feq (+ half of
+// Fbar's), r = ((1 - omega) * 0.25) a, f* = q +- r.
 static void mrt_collide(const oracle_case* c, const double* pn, double* po, double rho,
                         double ux, double uy, double fx, double fy) {
-  static const int PK[4] = {1, 2, 5, 6}, PB[4] = {3, 4, 7, 8};
   const double om = c->omega;
-  const bool forced = c->forcing != 0;
-  double feq[9], fb[9];
+  double q[9], e[9];
   for (int k = 0; k < 9; ++k) {
-    feq[k] = feq_k(c, k, rho, ux, uy);
-    fb[k] = forced ? guo_k(c, k, ux, uy, fx, fy) : 0.0;
+    q[k] = feq_k(c, k, rho, ux, uy);
+    e[k] = pn[k] - q[k];
   }
-  const double e0 = pn[0] - feq[0];
-  double eS[4], eD[4], fS[4], fD[4];
-  for (int n = 0; n < 4; ++n) {
-    const double ek = pn[PK[n]] - feq[PK[n]];
-    const double ekb = pn[PB[n]] - feq[PB[n]];
-    eS[n] = ek + ekb;
-    eD[n] = ek - ekb;
-    fS[n] = fb[PK[n]] + fb[PB[n]];
-    fD[n] = fb[PK[n]] - fb[PB[n]];
+  double a7 = (e[1] + e[3]) - (e[2] + e[4]);
+  double a8 = (e[5] + e[7]) - (e[6] + e[8]);
+  if (c->forcing != 0) {
+    double fb[9];
+    for (int k = 0; k < 9; ++k) fb[k] = guo_k(c, k, ux, uy, fx, fy);
+    a7 = a7 + 0.5 * ((fb[1] + fb[3]) - (fb[2] + fb[4]));
+    a8 = a8 + 0.5 * ((fb[5] + fb[7]) - (fb[6] + fb[8]));
+    for (int k = 0; k < 9; ++k) q[k] = q[k] + 0.5 * fb[k];
   }
-  double me[9], mf[9], gm[9], out[9];
-  pair_moments(e0, eS, eD, me);
-  pair_moments(fb[0], fS, fD, mf);
-  for (int i = 0; i < 9; ++i) {
-    const double s = (i >= 7) ? om : 1.0;
-    double p = (1.0 - s) * me[i];
-    if (forced) p = p + (1.0 - 0.5 * s) * mf[i];
-    gm[i] = p * (1.0 / (double)ROWNORM[i]);
-  }
-  pair_inverse(gm, out);
-  for (int k = 0; k < 9; ++k) po[k] = feq[k] + out[k];
+  const double cs = (1.0 - om) * 0.25;
+  const double r7 = cs * a7, r8 = cs * a8;
+  po[0] = q[0];
+  po[1] = q[1] + r7;
+  po[2] = q[2] - r7;
+  po[3] = q[3] + r7;
+  po[4] = q[4] - r7;
+  po[5] = q[5] + r8;
+  po[6] = q[6] - r8;
+  po[7] = q[7] + r8;
+  po[8] = q[8] - r8;
 }
 
 int oracle_initialize(oracle_case* c) {

@@ -51,16 +51,6 @@ __device__ constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 __device__ constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 __device__ constexpr int INV[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
 
-// Reference MRT moment matrix (base/collision_operator.py:147-157) and the
-// squared row norms that give M^-1 = M^T diag(1/norm).
-__device__ constexpr int MM[9][9] = {
-    {1, 1, 1, 1, 1, 1, 1, 1, 1},   {-4, -1, -1, -1, -1, 2, 2, 2, 2},
-    {4, -2, -2, -2, -2, 1, 1, 1, 1}, {0, 1, 0, -1, 0, 1, -1, -1, 1},
-    {0, -2, 0, 2, 0, 1, -1, -1, 1}, {0, 0, 1, 0, -1, 1, 1, -1, -1},
-    {0, 0, -2, 0, 2, 1, 1, -1, -1}, {0, 1, -1, 1, -1, 0, 0, 0, 0},
-    {0, 0, 0, 0, 0, 1, -1, 1, -1}};
-__device__ constexpr int ROWNORM[9] = {9, 36, 36, 6, 12, 6, 12, 4, 4};
-
 // ---------------------------------------------------------------------------
 // Scalar constants in the compute type, loaded once per thread from the
 // descriptor (double fields for FP64, the float mirror otherwise).
@@ -351,7 +341,7 @@ template <int FORCING>
 __device__ __forceinline__ void moments_f32(const float (&g)[9], const d2q9_lattice& L,
                                             const Consts<float>& K, float ffx, float ffy,
                                             float& drho, float& rho, float& ux, float& uy,
-                                            float& fx, float& fy) {
+                                            float& fx, float& fy, float& px, float& py) {
   // pairs of opposite directions first: fewer dependent adds
   const float s13 = g[1] + g[3], s24 = g[2] + g[4], s57 = g[5] + g[7], s68 = g[6] + g[8];
   const float d13 = g[1] - g[3], d24 = g[2] - g[4], d57 = g[5] - g[7], d68 = g[6] - g[8];
@@ -370,21 +360,23 @@ __device__ __forceinline__ void moments_f32(const float (&g)[9], const d2q9_latt
   const float sy = d24 + (d57 + d68);
   const float inv = __builtin_amdgcn_rcpf(rho + K.fmin);
   if (FORCING != D2Q9_FORCING_NONE || field) {
-    ux = __builtin_fmaf(0.5f, fx, sx) * inv;
-    uy = __builtin_fmaf(0.5f, fy, sy) * inv;
+    px = __builtin_fmaf(0.5f, fx, sx);
+    py = __builtin_fmaf(0.5f, fy, sy);
   } else {
-    ux = sx * inv;
-    uy = sy * inv;
+    px = sx;
+    py = sy;
   }
+  ux = px * inv;
+  uy = py * inv;
 }
 
 template <typename T, int FORCING, bool DEV>
 __device__ __forceinline__ void moments(const T (&g)[9], const d2q9_lattice& L,
                                         const Consts<T>& K, T ffx, T ffy,
                                         T& drho, T& rho, T& ux, T& uy, T& fx,
-                                        T& fy) {
+                                        T& fy, T& px, T& py) {
   if constexpr (std::is_same<T, float>::value && DEV) {
-    moments_f32<FORCING>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
+    moments_f32<FORCING>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy, px, py);
     return;
   }
   T r = (T)0;
@@ -429,73 +421,92 @@ __device__ __forceinline__ void moments(const T (&g)[9], const d2q9_lattice& L,
     ux = sx * inv;
     uy = sy * inv;
   }
+  // momentum sums (with half the force when forced); only the float
+  // collisions read them
+  px = sx + (T)0.5 * fx;
+  py = sy + (T)0.5 * fy;
 }
 
-// Float BGK (deviation form) over the four opposite-direction pairs.  With
-// cu_kb = -cu_k the relaxed equilibrium om geq splits into a part shared by
-// the pair, A = om w (drho + rho (0.5 ics4 cu^2 - 0.5 ics2 u^2)), and one of
-// opposite sign, B = om w rho ics2 cu; the Guo source (times 1 - om/2)
-// likewise into (1 - om/2) w (ics4 cu (c.F) - ics2 (u.F)) (second order
-// only) and (1 - om/2) w ics2 (c.F).  Then g_k <- (1 - om) g_k + (A + B),
-// g_kb <- (1 - om) g_kb + (A - B), with om w rho and om w drho formed once
-// per cell and weight.
+// Float collisions (deviation form) over the four opposite-direction
+// pairs.  The equilibrium is written so that its mass and momentum equal
+// the cell's up to the rounding of per-cell values: a constant factor of
+// its own rounding (fl(3 w) with 3 (2 w1 + 4 w5) != 1 in binary, a weight
+// sum != 1, fl(om w)) would put the same relative error into every update
+// of a conserved quantity, which the collision does not relax away: it
+// accumulates over the run (measured: FP32 cylinder 8192x2048 after 50,000
+// steps, mean flow off by 1.7e-3).  So
+//   geq_k + geq_kb = 2 w (drho + rho t_k),  t_k = 0.5 ics4 cu^2 - 0.5 ics2 u^2,
+//   geq_k - geq_kb = 2 m (c_k . P),  P = sum_k c_k g_k (+ F/2) = rho u,
+//   m1 = fl(ics2 w1) (axis pairs), m5 = (1 - 2 m1) / 4 (diagonal pairs),
+//     so that 2 m1 + 4 m5 = 1 exactly and the momentum is P itself,
+//   geq_0 = drho - sum_{k>0} geq_k  (mass drho),
+// and the same for the Guo source (its shared parts sum to minus the rest
+// direction's, its momentum is (1 - om/2) F through m).  omega multiplies
+// per-cell values (rho, drho, P), never a weight.
+struct FloatEq {
+  float hics4, hu2, m[2];
+};
+
+__device__ __forceinline__ FloatEq float_eq(const Consts<float>& K, float ux, float uy) {
+  FloatEq E;
+  E.hics4 = 0.5f * K.ics4;
+  E.hu2 = (0.5f * K.ics2) * __builtin_fmaf(ux, ux, uy * uy);
+  E.m[0] = K.ics2 * K.w[1];
+  E.m[1] = 0.25f * __builtin_fmaf(-2.0f, E.m[0], 1.0f);
+  return E;
+}
+
+// BGK: g_k <- (1 - om) g_k + om geq_k (+ (1 - om/2) Fbar_k).
 template <int FORCING>
 __device__ __forceinline__ void collide_bgk_pairs(float (&g)[9], const Consts<float>& K,
                                                   float drho, float rho, float ux, float uy,
-                                                  float fx, float fy) {
+                                                  float fx, float fy, float px, float py) {
   // fused multiply-adds are written out (never left to the compiler), so
   // every kernel build computes the same values
+  constexpr bool forced = FORCING != D2Q9_FORCING_NONE;
+  constexpr bool second = FORCING == D2Q9_GUO_SECOND_ORDER;
   const float om = K.om;
   const float omc = 1.0f - om;
-  const float ics2 = K.ics2;
-  const float hics4 = 0.5f * K.ics4;
-  const float hu2 = (0.5f * ics2) * __builtin_fmaf(ux, ux, uy * uy);
+  const FloatEq E = float_eq(K, ux, uy);
   const float cf = __builtin_fmaf(-0.5f, om, 1.0f);
-  float uf = 0.0f;
-  if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) uf = ics2 * __builtin_fmaf(ux, fx, uy * fy);
-  // rest direction (cu = 0, c = 0)
-  {
-    const float omw = om * K.w[0];
-    float a = __builtin_fmaf(-(rho * omw), hu2, drho * omw);
-    if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) a = __builtin_fmaf(cf * K.w[0], -uf, a);
-    g[0] = __builtin_fmaf(omc, g[0], a);
+  const float rom = rho * om, dom = drho * om;
+  const float opx = om * px, opy = om * py;
+  float cfx = 0.0f, cfy = 0.0f, uf = 0.0f;
+  if constexpr (forced) {
+    cfx = cf * fx;
+    cfy = cf * fy;
   }
+  if constexpr (second) uf = K.ics2 * __builtin_fmaf(ux, fx, uy * fy);
   constexpr int PK[4] = {1, 2, 5, 6};
   constexpr int PB[4] = {3, 4, 7, 8};
-  // per weight class (axis pairs 0, 1; diagonal pairs 2, 3)
-  float rw[2], dw[2], aw[2];
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const float omw = om * K.w[c ? 5 : 1];
-    rw[c] = rho * omw;
-    dw[c] = drho * omw;
-    aw[c] = rw[c] * ics2;
-  }
+  float shared = 0.0f;
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    const int k = PK[n], kb = PB[n], c = n >> 1;
+    const int k = PK[n], kb = PB[n];
+    const float w = K.w[k], m = E.m[n >> 1];
     const float cu = (float)CX[k] * ux + (float)CY[k] * uy;
-    const float t = __builtin_fmaf(hics4 * cu, cu, -hu2);
-    float A = __builtin_fmaf(rw[c], t, dw[c]);
-    float B = aw[c] * cu;
-    if constexpr (FORCING != D2Q9_FORCING_NONE) {
-      const float w = K.w[k];
+    const float t = __builtin_fmaf(E.hics4 * cu, cu, -E.hu2);
+    float A = w * __builtin_fmaf(rom, t, dom);
+    float B = m * ((float)CX[k] * opx + (float)CY[k] * opy);
+    if constexpr (forced) B = __builtin_fmaf(m, (float)CX[k] * cfx + (float)CY[k] * cfy, B);
+    if constexpr (second) {
       const float cF = (float)CX[k] * fx + (float)CY[k] * fy;
-      B = __builtin_fmaf((cf * w) * ics2, cF, B);
-      if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER)
-        A = __builtin_fmaf(cf * w, __builtin_fmaf(K.ics4 * cu, cF, -uf), A);
+      A = __builtin_fmaf(cf * w, __builtin_fmaf(K.ics4 * cu, cF, -uf), A);
     }
+    shared += A;
     g[k] = __builtin_fmaf(omc, g[k], A + B);
     g[kb] = __builtin_fmaf(omc, g[kb], A - B);
   }
+  g[0] = __builtin_fmaf(omc, g[0], __builtin_fmaf(-2.0f, shared, dom));
 }
 
 // BGK collision in place: g <- (1 - om) g + om geq (+ (1 - om/2) Fbar).
 template <typename T, int FORCING, bool DEV>
 __device__ __forceinline__ void collide_bgk(T (&g)[9], const Consts<T>& K,
-                                            T drho, T rho, T ux, T uy, T fx, T fy) {
+                                            T drho, T rho, T ux, T uy, T fx, T fy,
+                                            T px, T py) {
   if constexpr (std::is_same<T, float>::value && DEV) {
-    collide_bgk_pairs<FORCING>(g, K, drho, rho, ux, uy, fx, fy);
+    collide_bgk_pairs<FORCING>(g, K, drho, rho, ux, uy, fx, fy, px, py);
     return;
   }
   const T om = K.om;
@@ -514,219 +525,131 @@ __device__ __forceinline__ void collide_bgk(T (&g)[9], const Consts<T>& K,
   }
 }
 
-// The nine moments (rows of M) of a population-like vector given as its
-// rest value x0 and the sums S and differences D of the opposite pairs
-// (1,3), (2,4), (5,7), (6,8).
-__device__ __forceinline__ void moments_of_pairs(float x0, const float (&S)[4],
-                                                 const float (&D)[4], float (&m)[9]) {
-  const float s12 = S[0] + S[1], s34 = S[2] + S[3];
-  m[0] = x0 + s12 + s34;
-  m[1] = __builtin_fmaf(2.0f, s34, __builtin_fmaf(-4.0f, x0, -s12));
-  m[2] = __builtin_fmaf(-2.0f, s12, __builtin_fmaf(4.0f, x0, s34));
-  const float a = D[2] - D[3], b = D[2] + D[3];
-  m[3] = D[0] + a;
-  m[4] = __builtin_fmaf(-2.0f, D[0], a);
-  m[5] = D[1] + b;
-  m[6] = __builtin_fmaf(-2.0f, D[1], b);
-  m[7] = S[0] - S[1];
-  m[8] = S[2] - S[3];
-}
-
-// Float MRT (deviation form) with the same pair structure: the columns of
-// M for k and its opposite kb share the rows 0, 1, 2, 7, 8 and negate the
-// rows 3..6, so M (g - geq), M Fbar and the inverse map M^T diag(1/norm)
-// all go through pair sums and differences.  Same operator as the FP64
-// path, f* = f - M^-1 [S (M f - M feq) - (I - S/2) M Fbar], different
-// rounding.
+// MRT with the reference's rates S = (1,1,1,1,1,1,1,s7,s8), s7 = s8 =
+// omega (base/collision_operator.py:158-162).  The operator
+//   f* = feq + M^-1 [(I - S) M (f - feq) + (I - S/2) M Fbar]
+// has, with unit rates on rows 0..6, only the two stress rows left in the
+// first term, and M^-1 = M^T diag(1/|row|^2) with |row 7|^2 = |row 8|^2 = 4:
+//   f*_k = feq_k + 0.5 Fbar_k + sigma_k (1 - s) / 4 (p + 0.5 p^F),
+// p = p_xx = (e1 + e3) - (e2 + e4) of e = f - feq (row 7) for k = 1..4,
+// p_xy = (e5 + e7) - (e6 + e8) (row 8) for k = 5..8, p^F the same moment of
+// Fbar, sigma_k = M[7][k] or M[8][k] (+1 for 1, 3, 5, 7; -1 for 2, 4, 6, 8);
+// f*_0 = feq_0 + 0.5 Fbar_0.
+//
+// Float form (deviation storage), with the equilibrium and source of the
+// float BGK above (mass and momentum exact up to per-cell rounding).  The
+// stress moments of the equilibrium and the source have closed forms,
+//   p_xx(geq) = w1 rho ics4 (ux^2 - uy^2),  p_xy(geq) = 4 w5 rho ics4 ux uy,
+//   p_xx(Fbar) = 2 w1 ics4 (ux fx - uy fy), p_xy(Fbar) = 4 w5 ics4 (ux fy + uy fx)
+// (second-order Guo; the linear source has none).  The same operator as
+// the FP64 path, with other rounding.
 template <int FORCING>
 __device__ __forceinline__ void collide_mrt_pairs(float (&g)[9], const Consts<float>& K,
                                                   float drho, float rho, float ux, float uy,
-                                                  float fx, float fy) {
-  const float ics2 = K.ics2;
-  const float hics4 = 0.5f * K.ics4;
-  const float hu2 = (0.5f * ics2) * __builtin_fmaf(ux, ux, uy * uy);
-  const float rics2 = rho * ics2;
-  const float uf = ics2 * __builtin_fmaf(ux, fx, uy * fy);
+                                                  float fx, float fy, float px, float py) {
+  constexpr bool forced = FORCING != D2Q9_FORCING_NONE;
+  constexpr bool second = FORCING == D2Q9_GUO_SECOND_ORDER;
+  const float ics4 = K.ics4;
+  const FloatEq E = float_eq(K, ux, uy);
+  const float w1 = K.w[1], w5 = K.w[5];
+  const float rics4 = rho * ics4;
+  // stress moments of g - geq (+ half of Fbar's), times (1 - s) / 4
+  float pxx = ((g[1] + g[3]) - (g[2] + g[4])) -
+              (w1 * rics4) * __builtin_fmaf(ux, ux, -(uy * uy));
+  float pxy = ((g[5] + g[7]) - (g[6] + g[8])) - ((4.0f * w5) * rics4) * (ux * uy);
+  if constexpr (second) {
+    pxx = __builtin_fmaf(w1 * ics4, __builtin_fmaf(ux, fx, -(uy * fy)), pxx);
+    pxy = __builtin_fmaf((2.0f * w5) * ics4, __builtin_fmaf(ux, fy, uy * fx), pxy);
+  }
+  const float r7 = (0.25f * (1.0f - K.rates[7])) * pxx;
+  const float r8 = (0.25f * (1.0f - K.rates[8])) * pxy;
+  float uf = 0.0f;
+  if constexpr (second) uf = K.ics2 * __builtin_fmaf(ux, fx, uy * fy);
+  const float hfx = 0.5f * fx, hfy = 0.5f * fy;
   constexpr int PK[4] = {1, 2, 5, 6};
   constexpr int PB[4] = {3, 4, 7, 8};
-  // non-equilibrium part e = g - geq and the Guo source, as pair sums and
-  // differences
-  const float e0 = g[0] - K.w[0] * __builtin_fmaf(-rho, hu2, drho);
-  float eS[4], eD[4], fS[4], fD[4];
-  float f0 = 0.0f;
-  if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) f0 = -(K.w[0] * uf);
+  float shared = 0.0f;
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     const int k = PK[n], kb = PB[n];
+    const float w = K.w[k], m = E.m[n >> 1];
     const float cu = (float)CX[k] * ux + (float)CY[k] * uy;
-    const float w = K.w[k];
-    const float sym = __builtin_fmaf(rho, __builtin_fmaf(hics4 * cu, cu, -hu2), drho);
-    const float asym = rics2 * cu;
-    // geq_k + geq_kb = 2 w sym, geq_k - geq_kb = 2 w asym
-    eS[n] = (g[k] + g[kb]) - (2.0f * w) * sym;
-    eD[n] = (g[k] - g[kb]) - (2.0f * w) * asym;
-    fS[n] = 0.0f;
-    fD[n] = 0.0f;
-    if constexpr (FORCING != D2Q9_FORCING_NONE) {
+    const float t = __builtin_fmaf(E.hics4 * cu, cu, -E.hu2);
+    float A = w * __builtin_fmaf(rho, t, drho);
+    float B = m * ((float)CX[k] * px + (float)CY[k] * py);
+    if constexpr (forced) B = __builtin_fmaf(m, (float)CX[k] * hfx + (float)CY[k] * hfy, B);
+    if constexpr (second) {
       const float cF = (float)CX[k] * fx + (float)CY[k] * fy;
-      fD[n] = (2.0f * w) * (ics2 * cF);
-      if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER)
-        fS[n] = (2.0f * w) * __builtin_fmaf(K.ics4 * cu, cF, -uf);
+      A = __builtin_fmaf(0.5f * w, __builtin_fmaf(ics4 * cu, cF, -uf), A);
     }
+    shared += A;
+    const float r = n < 2 ? r7 : r8;
+    const float Ar = (n & 1) ? A - r : A + r;
+    g[k] = Ar + B;
+    g[kb] = Ar - B;
   }
-  float me[9], mf[9];
-  moments_of_pairs(e0, eS, eD, me);
-  if constexpr (FORCING != D2Q9_FORCING_NONE) moments_of_pairs(f0, fS, fD, mf);
-  float gm[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    const float sr = K.rates[i];
-    float dm = sr * me[i];
-    if constexpr (FORCING != D2Q9_FORCING_NONE)
-      dm = __builtin_fmaf(-__builtin_fmaf(-0.5f, sr, 1.0f), mf[i], dm);
-    gm[i] = dm * (float)(1.0 / (double)ROWNORM[i]);
-  }
-  // g -= M^T gm, by columns: rest, then per pair the shared and the
-  // opposite-signed part
-  g[0] -= __builtin_fmaf(4.0f, gm[2] - gm[1], gm[0]);
-  const float base13 = __builtin_fmaf(-2.0f, gm[2], gm[0] - gm[1]);
-  const float sym13 = base13 + gm[7], sym24 = base13 - gm[7];
-  const float asym13 = __builtin_fmaf(-2.0f, gm[4], gm[3]);
-  const float asym24 = __builtin_fmaf(-2.0f, gm[6], gm[5]);
-  const float base57 = __builtin_fmaf(2.0f, gm[1], gm[0] + gm[2]);
-  const float sym57 = base57 + gm[8], sym68 = base57 - gm[8];
-  const float p34 = gm[3] + gm[4], p56 = gm[5] + gm[6];
-  const float asym57 = p34 + p56, asym68 = p56 - p34;
-  g[1] -= sym13 + asym13;
-  g[3] -= sym13 - asym13;
-  g[2] -= sym24 + asym24;
-  g[4] -= sym24 - asym24;
-  g[5] -= sym57 + asym57;
-  g[7] -= sym57 - asym57;
-  g[6] -= sym68 + asym68;
-  g[8] -= sym68 - asym68;
+  g[0] = __builtin_fmaf(-2.0f, shared, drho);
 }
 
-// Moments (rows of M) of a population-like vector from its rest value x0
-// and the sums S and differences D of the pairs (1,3), (2,4), (5,7),
-// (6,8), without fused multiply-adds: the FP64 operation order of oracle
-// pair_moments.
-template <typename T>
-__device__ __forceinline__ void pair_moments(T x0, const T (&S)[4], const T (&D)[4],
-                                             T (&m)[9]) {
-  const T s12 = S[0] + S[1], s34 = S[2] + S[3];
-  const T a = D[2] - D[3], b = D[2] + D[3];
-  m[0] = (x0 + s12) + s34;
-  m[1] = (((T)-4 * x0) - s12) + (T)2 * s34;
-  m[2] = (((T)4 * x0) + s34) - (T)2 * s12;
-  m[3] = D[0] + a;
-  m[4] = a - (T)2 * D[0];
-  m[5] = D[1] + b;
-  m[6] = b - (T)2 * D[1];
-  m[7] = S[0] - S[1];
-  m[8] = S[2] - S[3];
-}
-
-// M^T gm by columns (oracle pair_inverse).
-template <typename T>
-__device__ __forceinline__ void pair_inverse(const T (&gm)[9], T (&out)[9]) {
-  out[0] = gm[0] + (T)4 * (gm[2] - gm[1]);
-  const T base13 = (gm[0] - gm[1]) - (T)2 * gm[2];
-  const T sym13 = base13 + gm[7], sym24 = base13 - gm[7];
-  const T asym13 = gm[3] - (T)2 * gm[4], asym24 = gm[5] - (T)2 * gm[6];
-  const T base57 = (gm[0] + gm[2]) + (T)2 * gm[1];
-  const T sym57 = base57 + gm[8], sym68 = base57 - gm[8];
-  const T p34 = gm[3] + gm[4], p56 = gm[5] + gm[6];
-  const T asym57 = p34 + p56, asym68 = p56 - p34;
-  out[1] = sym13 + asym13;
-  out[2] = sym24 + asym24;
-  out[3] = sym13 - asym13;
-  out[4] = sym24 - asym24;
-  out[5] = sym57 + asym57;
-  out[6] = sym68 + asym68;
-  out[7] = sym57 - asym57;
-  out[8] = sym68 - asym68;
-}
-
-// MRT with Guo forcing in place, FP64 storage: the operation order of
-// oracle mrt_collide,
-//   e = g - feq, Fbar = Guo source (pair sums and differences),
-//   p = (I - S) M e + (I - S/2) M Fbar,  g = feq + M^-1 p,
-// which equals g - M^-1 [S (M g - M feq) - (I - S/2) M Fbar].  The
-// populations are consumed pair by pair and feq is evaluated again for the
-// result instead of being held across the moments (the compiler is shown
-// copies of rho and u it cannot see through, so it cannot merge the two
-// evaluations): the live set stays near 24 doubles, which keeps the
-// two-step kernel at two workgroups per CU.
+// MRT, FP64 storage, in the operation order of the oracle's mrt_collide
+// (the reduced form above, direction by direction):
+//   q_k = feq_k (+ 0.5 Fbar_k),  e_k = f_k - feq_k,
+//   a7 = (e1 + e3) - (e2 + e4) (+ 0.5 ((F1 + F3) - (F2 + F4))),
+//   a8 = (e5 + e7) - (e6 + e8) (+ 0.5 ((F5 + F7) - (F6 + F8))),
+//   r = ((1 - s) * 0.25) a,  f*_k = q_k +- r (f*_0 = q_0).
 template <typename T, int FORCING, bool DEV>
 __device__ __forceinline__ void collide_mrt(T (&g)[9], const Consts<T>& K,
                                             T drho, T rho, T ux, T uy, T fx, T fy) {
   static_assert(!DEV, "deviation storage forms use collide_mrt_pairs");
   constexpr bool forced = FORCING != D2Q9_FORCING_NONE;
-  constexpr int PK[4] = {1, 2, 5, 6};
-  constexpr int PB[4] = {3, 4, 7, 8};
   const T ics2 = K.ics2, ics4 = K.ics4;
-  // moments of the Guo source first; the equilibrium part then starts from
-  // copies of rho and u that depend on them, so that the two sets of pair
-  // sums and differences are never live together
-  T mf[9];
+  const T u2 = ux * ux + uy * uy;
+  T q[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) q[k] = feq_k<T>(k, rho, ux, uy, u2, K.w[k], ics2, ics4);
+  T a7 = ((g[1] - q[1]) + (g[3] - q[3])) - ((g[2] - q[2]) + (g[4] - q[4]));
+  T a8 = ((g[5] - q[5]) + (g[7] - q[7])) - ((g[6] - q[6]) + (g[8] - q[8]));
   if constexpr (forced) {
-    T fS[4], fD[4];
-    const T f0 = guo_k<T, FORCING>(0, ux, uy, fx, fy, K.w[0], ics2, ics4);
+    // the source pair by pair (sum into the stress moment, half into q),
+    // so that no more than two source terms are live next to q
+    T ps[4];
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      const int k = PK[n], kb = PB[n];
+      const int k = n + 1 + (n >> 1) * 2, kb = k + 2;  // (1,3) (2,4) (5,7) (6,8)
       const T fk = guo_k<T, FORCING>(k, ux, uy, fx, fy, K.w[k], ics2, ics4);
       const T fkb = guo_k<T, FORCING>(kb, ux, uy, fx, fy, K.w[kb], ics2, ics4);
-      fS[n] = fk + fkb;
-      fD[n] = fk - fkb;
+      ps[n] = fk + fkb;
+      q[k] = q[k] + (T)0.5 * fk;
+      q[kb] = q[kb] + (T)0.5 * fkb;
     }
-    pair_moments<T>(f0, fS, fD, mf);
-    asm volatile("" : "+v"(rho), "+v"(ux), "+v"(uy)
-                 : "v"(mf[0]), "v"(mf[1]), "v"(mf[2]), "v"(mf[3]), "v"(mf[4]),
-                   "v"(mf[5]), "v"(mf[6]), "v"(mf[7]), "v"(mf[8]));
+    a7 = a7 + (T)0.5 * (ps[0] - ps[1]);
+    a8 = a8 + (T)0.5 * (ps[2] - ps[3]);
+    q[0] = q[0] + (T)0.5 * guo_k<T, FORCING>(0, ux, uy, fx, fy, K.w[0], ics2, ics4);
   }
-  const T u2 = ux * ux + uy * uy;
-  const T e0 = g[0] - feq_k<T>(0, rho, ux, uy, u2, K.w[0], ics2, ics4);
-  T eS[4], eD[4];
-#pragma unroll
-  for (int n = 0; n < 4; ++n) {
-    const int k = PK[n], kb = PB[n];
-    const T ek = g[k] - feq_k<T>(k, rho, ux, uy, u2, K.w[k], ics2, ics4);
-    const T ekb = g[kb] - feq_k<T>(kb, rho, ux, uy, u2, K.w[kb], ics2, ics4);
-    eS[n] = ek + ekb;
-    eD[n] = ek - ekb;
-  }
-  T me[9], gm[9];
-  pair_moments<T>(e0, eS, eD, me);
-#pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    const T s = K.rates[i];
-    T p = ((T)1 - s) * me[i];
-    if constexpr (forced) p = p + ((T)1 - (T)0.5 * s) * mf[i];
-    gm[i] = p * (T)(1.0 / (double)ROWNORM[i]);
-  }
-  T out[9];
-  pair_inverse<T>(gm, out);
-  T r2 = rho, x2 = ux, y2 = uy;
-  asm volatile("" : "+v"(r2), "+v"(x2), "+v"(y2));
-  const T u2b = x2 * x2 + y2 * y2;
-#pragma unroll
-  for (int k = 0; k < 9; ++k)
-    g[k] = feq_k<T>(k, r2, x2, y2, u2b, K.w[k], ics2, ics4) + out[k];
+  const T r7 = (((T)1 - K.rates[7]) * (T)0.25) * a7;
+  const T r8 = (((T)1 - K.rates[8]) * (T)0.25) * a8;
+  g[0] = q[0];
+  g[1] = q[1] + r7;
+  g[2] = q[2] - r7;
+  g[3] = q[3] + r7;
+  g[4] = q[4] - r7;
+  g[5] = q[5] + r8;
+  g[6] = q[6] - r8;
+  g[7] = q[7] + r8;
+  g[8] = q[8] - r8;
   (void)drho;
 }
 
 template <typename T, int MODEL, int FORCING, bool DEV>
 __device__ __forceinline__ void collide(T (&g)[9], const Consts<T>& K, T drho,
-                                        T rho, T ux, T uy, T fx, T fy) {
+                                        T rho, T ux, T uy, T fx, T fy, T px, T py) {
   if constexpr (MODEL == D2Q9_MRT) {
     if constexpr (std::is_same<T, float>::value && DEV)
-      collide_mrt_pairs<FORCING>(g, K, drho, rho, ux, uy, fx, fy);
+      collide_mrt_pairs<FORCING>(g, K, drho, rho, ux, uy, fx, fy, px, py);
     else
       collide_mrt<T, FORCING, DEV>(g, K, drho, rho, ux, uy, fx, fy);
   } else
-    collide_bgk<T, FORCING, DEV>(g, K, drho, rho, ux, uy, fx, fy);
+    collide_bgk<T, FORCING, DEV>(g, K, drho, rho, ux, uy, fx, fy, px, py);
 }
 
 // ---------------------------------------------------------------------------

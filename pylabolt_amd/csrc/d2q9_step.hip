@@ -154,9 +154,9 @@ __device__ __forceinline__ void deep_collide(const d2q9_lattice& L,
     ffx = ff[c];
     ffy = ff[L.plane + c];
   }
-  T drho, fx, fy;
-  moments<T, FORCING, Codec<S>::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
-  collide<T, MODEL, FORCING, Codec<S>::dev>(g, K, drho, rho, ux, uy, fx, fy);
+  T drho, fx, fy, px, py;
+  moments<T, FORCING, Codec<S>::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy, px, py);
+  collide<T, MODEL, FORCING, Codec<S>::dev>(g, K, drho, rho, ux, uy, fx, fy, px, py);
 }
 
 // Wall kinds of a deep tile at the low-y (j = 1) and high-y (j = ny) walls:
@@ -356,11 +356,11 @@ __global__ __launch_bounds__(256) void k_moments(const d2q9_lattice L,
     ffx = ff[c];
     ffy = ff[P + c];
   }
-  T drho, rho, ux, uy, fx, fy;
+  T drho, rho, ux, uy, fx, fy, px, py;
   if (L.forcing == D2Q9_FORCING_NONE)
-    moments<T, D2Q9_FORCING_NONE, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
+    moments<T, D2Q9_FORCING_NONE, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy, px, py);
   else
-    moments<T, D2Q9_GUO_LINEAR, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
+    moments<T, D2Q9_GUO_LINEAR, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy, px, py);
   mom[c] = rho;
   mom[P + c] = ux;
   mom[2 * P + c] = uy;
@@ -429,6 +429,11 @@ int validate(const d2q9_lattice* L) {
   }
   if (L->storage < 0 || L->storage > 3) return fail(D2Q9_E_ARG, "bad storage");
   if (L->collision < 0 || L->collision > 1) return fail(D2Q9_E_ARG, "bad collision");
+  if (L->collision == D2Q9_MRT)
+    for (int k = 0; k < 7; ++k)
+      if (L->rates[k] != 1.0 || L->cf.rates[k] != 1.0f)
+        return fail(D2Q9_E_UNSUPPORTED,
+                    "MRT relaxes rows 0..6 at rate 1 (the reference's S); only rates 7 and 8 are free");
   if (L->forcing < 0 || L->forcing > 2) return fail(D2Q9_E_ARG, "bad forcing");
   if (L->force_source == D2Q9_FSRC_FIELD && !L->force_field)
     return fail(D2Q9_E_ARG, "force_source FIELD needs force_field");

@@ -211,9 +211,9 @@ __device__ __forceinline__ void finish_cell(
       ffx = ff[c];
       ffy = ff[L.plane + c];
     }
-    T drho, rho, ux, uy, fx, fy;
-    moments<T, FORCING, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy);
-    collide<T, MODEL, FORCING, C::dev>(g, K, drho, rho, ux, uy, fx, fy);
+    T drho, rho, ux, uy, fx, fy, px, py;
+    moments<T, FORCING, C::dev>(g, L, K, ffx, ffy, drho, rho, ux, uy, fx, fy, px, py);
+    collide<T, MODEL, FORCING, C::dev>(g, K, drho, rho, ux, uy, fx, fy, px, py);
     if constexpr (ENC) {
       // 16-bit storage rounds stochastically (deterministic in the data)
       const uint32_t key = C::dev && sizeof(typename C::raw) == 2 ? cell_key((uint32_t)c) : 0u;

@@ -612,12 +612,22 @@ class D2Q9Engine:
         L.gravity[0], L.gravity[1] = float(gravity[0]), float(gravity[1])
         if rates is None:
             rates = [1, 1, 1, 1, 1, 1, 1, omega, omega]
+        if model == "MRT" and any(float(r) != 1.0 for r in rates[:7]):
+            # the kernels evaluate the reference's S, whose rows 0..6 relax
+            # at rate 1 (base/collision_operator.py:158-162)
+            raise ValueError("MRT rates of rows 0..6 must be 1; only the "
+                             "stress rows 7 and 8 take other rates")
         for k in range(9):
             L.rates[k] = float(rates[k])
         if dev_scale is None:
-            # FP16 keeps (f - w) / w * 1024 clear of its subnormal range
+            # FP16 keeps (f - w) / w * 1024 clear of its subnormal range;
+            # the scale is the power of two at or below 1024 / w, so that
+            # decoding (times 1 / scale) undoes it exactly: a scale whose
+            # inverse rounds would bias every decoded value by the same
+            # relative amount, step after step
             if self.storage == nt.STORE_F16DEV:
-                dev_scale = [1024.0 / float(w) for w in self.weights]
+                dev_scale = [2.0 ** math.floor(math.log2(1024.0 / float(w)))
+                             for w in self.weights]
             else:
                 dev_scale = [1.0] * 9
         for k in range(9):

@@ -155,7 +155,13 @@ def test_boundaries_fp64_exact(name):
             assert v <= 1e-12, f"{name} step {done}: {k} {v}"
 
 
-ABS_TOL = {"f32": 2e-6, "f16dev": 1e-4, "bf16dev": 1e-3}
+# Max absolute error over every fluid cell and direction after 20 steps of
+# a rough field (|u| up to 0.042, moving walls up to 0.1).  FP16: the
+# largest deviations (w |3 c.u| up to ~0.03) sit on a grid of at most
+# 2^-16 ~ 1.5e-5; 20 independent stochastic roundings of up to one grid
+# step each, with the tail of the max over ~10^4 values, stay within
+# 2e-4.  BF16 has 3 bits fewer.
+ABS_TOL = {"f32": 2e-6, "f16dev": 2e-4, "bf16dev": 1e-3}
 
 
 @pytest.mark.parametrize("storage", sorted(ABS_TOL))

@@ -197,3 +197,34 @@ def test_channel_full_size_conserves_mass():
     print(f"[mass] channel 4096x1024 1000 steps: {m0!r} -> {m1!r} rel {rel:.2e}",
           flush=True)
     assert rel <= 1e-12
+
+
+
+def test_mrt_rates_outside_the_reference_form_are_refused():
+    """The kernels evaluate the reference's S = (1,..,1,omega,omega): the
+    C ABI returns D2Q9_E_UNSUPPORTED (-3) for a descriptor with another
+    rate on rows 0..6, before any launch, takes other stress rates (rows 7
+    and 8), and the engine refuses such rates with ValueError."""
+    import ctypes
+    from pylabolt_amd import _native as nt
+    from pylabolt_amd.solvers.fluidLB import Solver
+    sim = cases.lid_driven_cavity(12, 10, model="MRT")
+    eng = _engine(sim).engine
+
+    def step(L):
+        return eng.lib.d2q9_fused_step(
+            ctypes.byref(L), nt.ptr(eng.f[0]), nt.ptr(eng.f[1]),
+            nt.ptr(eng.mom[0]), nt.ptr(eng.mom[1]), 0, eng._stream_ptr)
+    bad = nt.Lattice.from_buffer_copy(eng.L)
+    bad.rates[5] = bad.omega
+    assert step(bad) == -3
+    assert b"rows 0..6" in eng.lib.d2q9_last_error()
+    ok = nt.Lattice.from_buffer_copy(eng.L)
+    ok.rates[7] = ok.rates[8] = 1.5
+    ok.cf.rates[7] = ok.cf.rates[8] = 1.5
+    assert step(ok) == 0
+    eng.synchronize()
+    s = Solver(simulation=sim, verbose=False)
+    s.collision_operator.S[5] = s.collision_operator.omega_fluid
+    with pytest.raises(ValueError, match="rows 0..6"):
+        s.set_backend()

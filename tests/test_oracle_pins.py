@@ -11,9 +11,11 @@ the mathematics fixes, and a planted mistake of each kind fails:
   the linear form and u F + F u for the second-order one), from
   collision_kernels.py:48-153;
 * MRT with S = (1,1,1,1,1,1,1,omega,omega) (collision_operator.py:136-165):
-  its pair formulas are the reference's moment matrix M and its transpose,
-  and viscosity comes from the two stress rows only, so a forced channel
-  with tau != 1 follows the start-up series of nu = cs^2 (1/omega - 1/2);
+  the full moment map's pair formulas are the reference's moment matrix M
+  and its transpose, the reduced form the time step uses equals that map
+  to round-off, and viscosity comes from the two stress rows only, so a
+  forced channel with tau != 1 follows the start-up series of
+  nu = cs^2 (1/omega - 1/2);
 * zero-gradient: a field independent of y evolves between zero-gradient
   walls at y exactly as in a box periodic in y;
 * anti-bounce-back pressure: a uniform flow at the imposed density is a
@@ -157,15 +159,92 @@ def test_planted_mrt_pair_sign_is_caught():
 def test_planted_mrt_rates_on_wrong_rows_are_caught(monkeypatch):
     """omega on rows 5 and 6 (j_y, q_y) and unit rates on the stress rows:
     the channel then has nu = 1/6 and leaves the series."""
-    good = O.mrt_collide
-
-    def bad(case, rates=None):
+    def bad(case):
         om = case.omega
-        good(case, rates=np.array([1, 1, 1, 1, 1, om, om, 1, 1],
-                                  dtype=np.float64))
+        O.mrt_collide_general(case, rates=np.array(
+            [1, 1, 1, 1, 1, om, om, 1, 1], dtype=np.float64))
     monkeypatch.setattr(O, "mrt_collide", bad)
     ux, ref, umax = _mrt_channel(1500)
     assert np.max(np.abs(ux - ref)) > 5e-2 * umax
+
+
+def _mrt_state(forcing, seed=9):
+    """A perturbed, forced state on a small periodic box, moments computed
+    as the time step computes them."""
+    sim = cases.poiseuille_channel(9, 7, model="MRT", forcing=forcing)
+    sim.initial_fields_dict["default"]["fluid"]["velocity"] = {
+        "type": "func", "func": lambda i, j: (0.04 * np.sin(1.3 * i + j),
+                                              0.05 * np.cos(i - 1.9 * j))}
+    case = oracle_case(sim)
+    case.omega = np.float64(1.37)
+    case.gravity = np.array([2e-4, -3e-4])
+    O.initialize_pop(case)
+    rng = np.random.default_rng(seed)
+    case.pop_new[case.fluid] *= 1 + 0.02 * rng.standard_normal(
+        case.pop_new[case.fluid].shape)
+    O.compute_density(case)
+    O.compute_force(case)
+    O.compute_velocity(case)
+    return case
+
+
+def _reduced_vs_general(collide, forcing):
+    case = _mrt_state(forcing)
+    collide(case)
+    got = case.pop[case.fluid].copy()
+    O.mrt_collide_general(case)
+    return float(np.max(np.abs(got - case.pop[case.fluid])))
+
+
+@pytest.mark.parametrize("forcing", [None, "guo_linear", "guo_second_order"])
+def test_mrt_reduced_form_equals_full_moment_map(forcing):
+    """The time step's MRT (feq + Fbar / 2 + relaxed stress moments) is the
+    operator of the full moment map with S = (1,..,1,omega,omega), to
+    round-off (populations ~0.1, so 1e-16 is a few ulps)."""
+    err = _reduced_vs_general(O.mrt_collide, forcing)
+    print(forcing, err)
+    assert err <= 1e-16
+
+
+@pytest.mark.parametrize("plant", ["sign_r8", "quarter", "no_half_source"])
+def test_planted_mrt_reduced_form_mistakes_are_caught(plant):
+    """A wrong sign of the p_xy column, 1/2 in place of 1/|row|^2 = 1/4, or
+    the source's own half left out of the stress term: each differs from
+    the full moment map by far more than round-off."""
+    def bad(case):
+        O.mrt_collide(case)
+        m = case.fluid
+        om = case.omega
+        pn = case.pop_new[m]
+        rho = case.density[m]
+        ux, uy = case.velocity[m, 0], case.velocity[m, 1]
+        c = case.const
+        q = [O.equilibrium(rho, ux, uy, k, c) for k in range(9)]
+        a8 = (pn[:, 5] - q[5] + pn[:, 7] - q[7]) - (pn[:, 6] - q[6] +
+                                                    pn[:, 8] - q[8])
+        a7 = (pn[:, 1] - q[1] + pn[:, 3] - q[3]) - (pn[:, 2] - q[2] +
+                                                    pn[:, 4] - q[4])
+        fx, fy = case.force_field[m, 0], case.force_field[m, 1]
+        fb = [O.guo_source(k, ux, uy, fx, fy, case.forcing, c)
+              for k in range(9)]
+        if plant == "sign_r8":
+            r8 = (1 - om) * 0.25 * (a8 + 0.5 * ((fb[5] + fb[7]) -
+                                                (fb[6] + fb[8])))
+            for k in (5, 6, 7, 8):
+                case.pop[m, k] -= (2 if k in (5, 7) else -2) * r8
+        elif plant == "quarter":
+            r7 = (1 - om) * 0.25 * (a7 + 0.5 * ((fb[1] + fb[3]) -
+                                                (fb[2] + fb[4])))
+            for k in (1, 2, 3, 4):
+                case.pop[m, k] += (1 if k in (1, 3) else -1) * r7
+        else:
+            b7 = 0.5 * ((fb[1] + fb[3]) - (fb[2] + fb[4]))
+            for k in (1, 2, 3, 4):
+                case.pop[m, k] -= (1 if k in (1, 3) else -1) * \
+                    (1 - om) * 0.25 * b7
+    err = _reduced_vs_general(bad, "guo_second_order")
+    print(plant, err)
+    assert err > 1e-9
 
 
 # ---- zero-gradient -------------------------------------------------------------
