@@ -165,7 +165,9 @@ typedef struct d2q9_lattice {
   const int32_t* bc_slot;       /* [plane], read where D2Q9_CELL_BC */
   const d2q9_bc_node* bc_nodes;
   const d2q9_bc_elem* bc_elems;
-  const void* force_field;      /* [2][plane] when force_source == FIELD */
+  const void* force_field;      /* [2][plane] when force_source == FIELD
+                                 * (needs a Guo forcing: D2Q9_E_UNSUPPORTED
+                                 * with forcing NONE) */
   /* Wall cells swept inside bulk tiles (null: none).  wall_index[p] is the
    * slot of bulk tile p in wall_masks, or -1 when the tile has no such
    * cell.  For slot t and row r, words [((t * tile_i + r) * 6 + s) * cpl,
@@ -280,7 +282,8 @@ int d2q9_residues(const d2q9_lattice* L, const void* mom, void* mom_old,
 /* Plain copy kernel (bandwidth reference). */
 int d2q9_copy(void* dst, const void* src, int64_t bytes, void* stream);
 
-/* HIP graph capture of the work issued on `stream` between begin and end. */
+/* HIP graph capture of the work issued on `stream` between begin and end;
+ * end instantiates the graph and uploads it to the device (on `stream`). */
 int d2q9_graph_begin(void* stream);
 int d2q9_graph_end(void* stream, void** graph_exec);
 int d2q9_graph_launch(void* graph_exec, void* stream);

@@ -23,7 +23,7 @@ The adjoint of a step needs the forward state only through that step's
 rho/u (the collision is a function of the moments and the populations
 enter it linearly), which is what makes the moment store sufficient.
 
-Supported cases: BGK with Guo forcing (linear or second order) or none, a
+Supported cases: BGK with Guo forcing (linear or second order), a
 per-cell force field, bounce-back / periodic boundaries and static solids at
 rest -- for these, streaming permutes the populations and its transpose is
 an exact scatter.  The constructor rejects anything else.
@@ -61,6 +61,9 @@ class FlowRollout:
         co = CollisionOperator(simulation, FluidLB(), st, verbose=False)
         if co.collision_fluid != "BGK":
             raise ValueError("the adjoint supports the BGK collision model")
+        if co.forcing_fluid is None:
+            raise ValueError("the adjoint needs a Guo forcing model: the "
+                             "force field acts through its source term")
         for e in st.boundary.boundary_elements:
             if e.type_fluid not in ("bounce_back", "periodic"):
                 raise ValueError("the adjoint supports bounce_back and periodic "

@@ -149,7 +149,7 @@ __device__ __forceinline__ void deep_collide(const d2q9_lattice& L,
                                              typename Codec<S>::T& uy) {
   using T = typename Codec<S>::T;
   T ffx = (T)0, ffy = (T)0;
-  if (L.force_source == D2Q9_FSRC_FIELD) {
+  if (FORCING != D2Q9_FORCING_NONE && L.force_source == D2Q9_FSRC_FIELD) {
     const T* ff = static_cast<const T*>(L.force_field);
     ffx = ff[c];
     ffy = ff[L.plane + c];
@@ -179,12 +179,9 @@ __device__ __forceinline__ void deep_wall_global(const d2q9_lattice& L,
 }
 
 // Two workgroups (32 waves) per CU need 8 waves per SIMD, i.e. at most 64
-// VGPRs.  The float forms are held to that (MRT with Guo forcing: 67-84
-// VGPRs unbounded; bounded, at most 16 bytes per lane spill, and the FP32
-// 4096x1024 MRT channel measured 41.4 us per step against 51.0).  FP64
-// keeps the default for a 1024-thread block (4 waves per SIMD): its MRT with
-// Guo forcing needs 94 VGPRs and, held to 64, spilled 196 bytes per lane
-// and ran 3x slower.
+// VGPRs, and every form is held to that: BGK takes 43-62 VGPRs, the
+// reduced MRT 40-64 (FP64 MRT with second-order Guo at 64, with two spill
+// instructions).
 template <int S, int MODEL, int FORCING>
 __global__ __launch_bounds__(64 * DEEP_WAVES, D2Q9_DEEP_WAVES_PER_SIMD) void k_deep2(
     const d2q9_lattice L, const int32_t* __restrict__ deep, const uint8_t* __restrict__ walls,
@@ -437,6 +434,8 @@ int validate(const d2q9_lattice* L) {
   if (L->forcing < 0 || L->forcing > 2) return fail(D2Q9_E_ARG, "bad forcing");
   if (L->force_source == D2Q9_FSRC_FIELD && !L->force_field)
     return fail(D2Q9_E_ARG, "force_source FIELD needs force_field");
+  if (L->force_source == D2Q9_FSRC_FIELD && L->forcing == D2Q9_FORCING_NONE)
+    return fail(D2Q9_E_UNSUPPORTED, "a force field needs a Guo forcing");
   if (!L->flags) return fail(D2Q9_E_ARG, "null flags");
   if ((L->bc_nodes == nullptr) != (L->bc_slot == nullptr))
     return fail(D2Q9_E_ARG, "bc_nodes and bc_slot go together");
@@ -829,6 +828,11 @@ int d2q9_graph_end(void* stream, void** graph_exec) {
   int r = check_hip(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
   (void)hipGraphDestroy(g);
   if (r) return r;
+  // upload now, so that the first launch does not pay it
+  if ((r = check_hip(hipGraphUpload(ge, static_cast<hipStream_t>(stream)), "hipGraphUpload"))) {
+    (void)hipGraphExecDestroy(ge);
+    return r;
+  }
   *graph_exec = ge;
   return 0;
 }

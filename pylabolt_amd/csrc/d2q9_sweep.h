@@ -206,7 +206,7 @@ __device__ __forceinline__ void finish_cell(
     for (int k = 0; k < 9; ++k) out[k][q] = C::enc(g[k], k, K);
   } else {
     T ffx = (T)0, ffy = (T)0;
-    if (L.force_source == D2Q9_FSRC_FIELD) {
+    if (FORCING != D2Q9_FORCING_NONE && L.force_source == D2Q9_FSRC_FIELD) {
       const T* ff = static_cast<const T*>(L.force_field);
       ffx = ff[c];
       ffy = ff[L.plane + c];

@@ -541,6 +541,9 @@ class D2Q9Engine:
                  force_field=None, tile=(64, 4), rates=None, dev_scale=None,
                  device=None, graph_steps=16, stream=None, solid_tiles=None,
                  two_step=None):
+        if force_field is not None and forcing is None:
+            # a body force acts through the Guo source term
+            raise ValueError("a force field needs a Guo forcing model")
         self.lib = nt.lib()
         self.device = (torch.device(device) if device is not None else
                        torch.device("cuda", torch.cuda.current_device()))

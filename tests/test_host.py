@@ -303,3 +303,20 @@ def test_engine_refuses_without_gpu():
     _native._lib = None
     with pytest.raises(_native.NativeError, match="no HIP device"):
         _native.lib()
+
+
+def test_force_field_without_guo_forcing_is_refused():
+    """A body force acts through the Guo source term: the engine and the
+    adjoint refuse a force field with forcing None before touching the GPU
+    (the C ABI returns D2Q9_E_UNSUPPORTED for such a descriptor)."""
+    from pylabolt_amd import cases
+    from pylabolt_amd.adjoint import FlowRollout
+    from pylabolt_amd.engine import D2Q9Engine
+    w = [4 / 9] + [1 / 9] * 4 + [1 / 36] * 4
+    with pytest.raises(ValueError, match="Guo forcing"):
+        D2Q9Engine(4, 4, np.zeros(36, bool), np.ones(36), np.zeros((36, 2)),
+                   [], True, True, "BGK", None, 1.0, (0.0, 0.0), 3.0, 9.0, w,
+                   1e-16, force_field=np.zeros((36, 2)))
+    sim, _, _, _ = cases.adjoint_channel(8, 6, steps=4, forcing=None)
+    with pytest.raises(ValueError, match="Guo forcing"):
+        FlowRollout(sim, 4)
