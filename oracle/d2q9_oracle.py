@@ -327,9 +327,11 @@ def mrt_collide_general(case, rates=None):
     case.pop[m_] = res
 
 
-def mrt_collide(case):
+def mrt_collide(case, stress_rates=None):
     """Build-defined MRT with Guo forcing and the reference's rates
-    S = (1,1,1,1,1,1,1,omega,omega) (base/collision_operator.py:158-162).
+    S = (1,1,1,1,1,1,1,omega,omega) (base/collision_operator.py:158-162);
+    ``stress_rates`` = (s7, s8) replaces the two omegas (the engine takes
+    any rates on the stress rows).
 
     With unit rates on rows 0..6 only the stress rows 7 (p_xx) and 8 (p_xy)
     keep a part of f - feq, and M^-1 = M^T diag(1/|row|^2) with
@@ -340,7 +342,7 @@ def mrt_collide(case):
       e_k = f_k - feq_k
       a7 = (e1 + e3) - (e2 + e4) + 0.5 ((F1 + F3) - (F2 + F4))
       a8 = (e5 + e7) - (e6 + e8) + 0.5 ((F5 + F7) - (F6 + F8))
-      r = ((1 - omega) * 0.25) * a
+      r7 = ((1 - s7) * 0.25) * a7,  r8 = ((1 - s8) * 0.25) * a8
       f*_0 = q_0;  f*_k = q_k + r7 (k = 1, 3), - r7 (2, 4),
                           + r8 (5, 7), - r8 (6, 8)
 
@@ -362,9 +364,9 @@ def mrt_collide(case):
         a7 = a7 + 0.5 * ((fb[1] + fb[3]) - (fb[2] + fb[4]))
         a8 = a8 + 0.5 * ((fb[5] + fb[7]) - (fb[6] + fb[8]))
         q = [q[k] + 0.5 * fb[k] for k in range(9)]
-    cs = (1.0 - om) * 0.25
-    r7 = cs * a7
-    r8 = cs * a8
+    s7, s8 = (om, om) if stress_rates is None else stress_rates
+    r7 = ((1.0 - np.float64(s7)) * 0.25) * a7
+    r8 = ((1.0 - np.float64(s8)) * 0.25) * a8
     res = np.empty_like(pn)
     res[:, 0] = q[0]
     for k, r in ((1, r7), (3, r7), (5, r8), (7, r8)):

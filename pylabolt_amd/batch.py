@@ -163,8 +163,10 @@ class BatchedLattices:
         self.storage = storage
         self.solvers = []
         for sim in simulations:
+            # the batched launches step every lattice one step at a time:
+            # no two-step scratch lattice
             s = Solver(simulation=sim, storage=storage, verbose=False,
-                       stream=self.stream)
+                       stream=self.stream, two_step=False)
             s.set_backend(verbose=False)
             self.solvers.append(s)
         self.engines = [s.engine for s in self.solvers]

@@ -228,3 +228,24 @@ def test_mrt_rates_outside_the_reference_form_are_refused():
     s.collision_operator.S[5] = s.collision_operator.omega_fluid
     with pytest.raises(ValueError, match="rows 0..6"):
         s.set_backend()
+
+
+def test_mrt_free_stress_rates_bitwise(monkeypatch):
+    """Stress rates s7 = 1.3, s8 = 1.7 (the C ABI leaves rows 7 and 8
+    free): the engine against the oracle's reduced MRT with those rates,
+    forced channel with a y force, 30 steps, bitwise bar 1e-12."""
+    from pylabolt_amd.solvers.fluidLB import Solver
+    sim = cases.poiseuille_channel(24, 16, model="MRT",
+                                   forcing="guo_second_order")
+    sim.forcing_dict["gravity"] = [3e-5, 1e-5]
+    s = Solver(simulation=sim, verbose=False)
+    s.collision_operator.S[7], s.collision_operator.S[8] = 1.3, 1.7
+    s.set_backend()
+    case = oracle_case(sim)
+    good = O.mrt_collide
+    monkeypatch.setattr(O, "mrt_collide",
+                        lambda c: good(c, stress_rates=(1.3, 1.7)))
+    O.initialize_pop(case)
+    s.advance(30)
+    O.run(case, 30)
+    _compare(s, case, "MRT s7=1.3 s8=1.7")

@@ -206,6 +206,18 @@ def test_mrt_reduced_form_equals_full_moment_map(forcing):
     assert err <= 1e-16
 
 
+def test_mrt_reduced_form_with_other_stress_rates():
+    """Stress rates s7 != s8 (free in the engine): the reduced form with
+    them equals the full moment map with S = (1,..,1,s7,s8)."""
+    case = _mrt_state("guo_second_order")
+    O.mrt_collide(case, stress_rates=(1.3, 1.7))
+    got = case.pop[case.fluid].copy()
+    O.mrt_collide_general(case, rates=np.array([1, 1, 1, 1, 1, 1, 1, 1.3, 1.7]))
+    err = float(np.max(np.abs(got - case.pop[case.fluid])))
+    print(err)
+    assert err <= 1e-16
+
+
 @pytest.mark.parametrize("plant", ["sign_r8", "quarter", "no_half_source"])
 def test_planted_mrt_reduced_form_mistakes_are_caught(plant):
     """A wrong sign of the p_xy column, 1/2 in place of 1/|row|^2 = 1/4, or
