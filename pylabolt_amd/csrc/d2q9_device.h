@@ -113,27 +113,22 @@ __device__ __forceinline__ Consts<float> load_consts<float>(const d2q9_lattice& 
 // CPL: cells per lane (FP64 1: 8 bytes per lane and direction; FP32 2:
 // 8 bytes; 16-bit 2: 4 bytes).
 // ---------------------------------------------------------------------------
-// 32-bit integer hash ("lowbias32" finaliser, C. Wellons, 2018): xor-shift
-// and multiply rounds with full avalanche.
-__device__ __forceinline__ uint32_t mix32(uint32_t x) {
-  x ^= x >> 16;
-  x *= 0x7feb352du;
-  x ^= x >> 15;
-  x *= 0x846ca68bu;
-  x ^= x >> 16;
-  return x;
-}
-
 // Random bits for rounding value `b` (float bits) of direction k of cell
 // `salt`: a function of the data only, so a run is reproducible bit for
 // bit, while the rounding of a slowly changing field is decorrelated from
-// step to step.  The cell's key is hashed once per cell (cell_key); per
-// value one 24-bit multiply (full rate) spreads the low 24 bits of
-// value ^ key into the upper bits of the product, which the conversions
-// below use: bits 16..31 for bfloat16, bits 19..31 for half.
-__device__ __forceinline__ uint32_t cell_key(uint32_t salt) { return mix32(salt); }
+// step to step.  The cell's key is hashed once per cell (cell_key, from
+// full-rate 24-bit multiplies: a 32-bit multiply is quarter rate); per
+// value one xor-add with the direction's salt (v_xad_u32) and one 24-bit
+// multiply spread the low 24 bits of (value ^ key) + salt into the upper
+// bits of the product, which the conversions below use: bits 16..31 for
+// bfloat16, bits 19..31 for half.
+__device__ __forceinline__ uint32_t cell_key(uint32_t salt) {
+  uint32_t x = __umul24(salt, 0x9e3779u) ^ (salt >> 13);
+  x = __umul24(x ^ (x >> 16), 0x85ebcau);
+  return x ^ (x >> 15);
+}
 __device__ __forceinline__ uint32_t round_bits(uint32_t b, int k, uint32_t key) {
-  const uint32_t x = b ^ (key + (uint32_t)k * 0x632be5abu);
+  const uint32_t x = (b ^ key) + (uint32_t)k * 0x632be5abu;
   return __umul24(x, 0x9e3779u);
 }
 

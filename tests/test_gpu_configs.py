@@ -135,7 +135,10 @@ def test_batched_256_lattices_fp32_bitwise_per_lattice():
     torch.cuda.synchronize()
 
     def fluid_pops(e, idx):
-        # the stored words of the fluid cells, compared on the device
+        # the stored words of the fluid cells, compared on the device once
+        # the engine's stream has written them (materialize() launches on
+        # it; these reads run on torch's current stream)
+        e.synchronize()
         aos = pop_aos(e.f[idx], e.nx, e.ny, e.B, e.nbj)
         keep = torch.from_numpy(e.tables.fluid_pad.ravel()).to(aos.device)
         return aos[keep]
