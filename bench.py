@@ -39,12 +39,16 @@ form: MLUPS and roofline of each, velocity error against the FP64 run and
 of every form against the start-up series of plane Poiseuille flow;
 ``batched``, 256 seeded lattices of uneven size advanced by one persistent
 launch per step; ``adjoint``, the forward + backward cost of the adjoint
-channel over its plain forward (one graph of all its steps); and a plain
-copy kernel's bandwidth on the same device.
+channel over its plain forward (one graph of all its steps), run in a
+child process (``ChildLeg``) so that a failure there cannot take the
+benchmark line with it; and a plain copy kernel's bandwidth on the same
+device.
 """
 import argparse
+import importlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -449,11 +453,56 @@ def channel_leg(storages, steps=None, warmup=100):
     return out
 
 
+class ChildLeg:
+    """One leg of the benchmark (``module:function`` returning a JSON-able
+    dict) in a child process.  The child is started before this process
+    touches the GPU and waits for a line on stdin; ``run`` sends it, waits
+    for the child's JSON record and returns it, or an ``error`` record when
+    the child fails (an abort there ends the child, not the benchmark
+    line).  Closing the pipe without ``run`` lets the child exit unused."""
+
+    def __init__(self, target):
+        self.target = target
+        self.proc = subprocess.Popen(
+            [sys.executable, os.path.abspath(__file__), "--child-leg", target],
+            stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
+
+    def run(self, timeout=900):
+        try:
+            out, _ = self.proc.communicate("go\n", timeout=timeout)
+        except subprocess.TimeoutExpired:
+            self.proc.kill()
+            self.proc.communicate()
+            return {"error": f"{self.target}: no result in {timeout} s"}
+        recs = [ln for ln in out.splitlines() if ln.startswith("{")]
+        if self.proc.returncode != 0 or not recs:
+            return {"error": f"{self.target}: child exited with status "
+                             f"{self.proc.returncode}"}
+        return json.loads(recs[-1])
+
+    def close(self):
+        if self.proc.poll() is None:
+            self.proc.communicate("", timeout=60)
+
+
+def child_leg(target, real_stdout):
+    """Child side of ``ChildLeg``: wait for the go line, run the leg, write
+    its record as one JSON line."""
+    if sys.stdin.readline().strip() != "go":
+        return
+    mod, fn = target.split(":")
+    out = getattr(importlib.import_module(mod), fn)()
+    print(json.dumps(out), file=real_stdout, flush=True)
+
+
 def main():
     # everything but the final JSON line goes to stderr
     real_stdout = sys.stdout
     sys.stdout = sys.stderr
     try:
+        if len(sys.argv) == 3 and sys.argv[1] == "--child-leg":
+            child_leg(sys.argv[2], real_stdout)
+            return
         out = run(real_stdout)
     finally:
         sys.stdout = real_stdout
@@ -492,6 +541,17 @@ def run(real_stdout):
                           "be 1 (multi-GPU sharding is a design note only)"}),
               file=real_stdout)
         sys.exit(2)
+    # the adjoint leg's child starts before this process touches the GPU
+    adj_leg = (ChildLeg("bench:adjoint_cost")
+               if not (args.no_adjoint or args.profile_run) else None)
+    try:
+        return run_legs(args, adj_leg)
+    finally:
+        if adj_leg is not None:
+            adj_leg.close()
+
+
+def run_legs(args, adj_leg):
     import torch
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
@@ -582,12 +642,16 @@ def run(real_stdout):
             f"{bat['per_lattice_launches_mlups']:.0f} MLUPS")
         torch.cuda.empty_cache()
     adj = None
-    if not args.no_adjoint:
-        adj = adjoint_cost()
-        log(f"adjoint: plain fwd {adj['plain_forward_ms']:.1f} ms, sqrt "
-            f"ratio {adj['sqrt']['cost_ratio']:.2f}, moments ratio "
-            f"{adj['moments']['cost_ratio']:.2f}")
+    if adj_leg is not None:
+        torch.cuda.synchronize()
         torch.cuda.empty_cache()
+        adj = adj_leg.run()
+        if "error" in adj:
+            log(f"adjoint: {adj['error']}")
+        else:
+            log(f"adjoint: plain fwd {adj['plain_forward_ms']:.1f} ms, sqrt "
+                f"ratio {adj['sqrt']['cost_ratio']:.2f}, moments ratio "
+                f"{adj['moments']['cost_ratio']:.2f}")
     copy_gbs = None if args.no_copy else copy_bandwidth()
     if copy_gbs:
         log(f"copy kernel: {copy_gbs:.0f} GB/s")

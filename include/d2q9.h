@@ -282,8 +282,8 @@ int d2q9_residues(const d2q9_lattice* L, const void* mom, void* mom_old,
 /* Plain copy kernel (bandwidth reference). */
 int d2q9_copy(void* dst, const void* src, int64_t bytes, void* stream);
 
-/* HIP graph capture of the work issued on `stream` between begin and end;
- * end instantiates the graph and uploads it to the device (on `stream`). */
+/* HIP graph capture of the work issued on `stream` between begin and end
+ * (end instantiates the graph). */
 int d2q9_graph_begin(void* stream);
 int d2q9_graph_end(void* stream, void** graph_exec);
 int d2q9_graph_launch(void* graph_exec, void* stream);

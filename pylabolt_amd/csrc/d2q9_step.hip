@@ -828,11 +828,6 @@ int d2q9_graph_end(void* stream, void** graph_exec) {
   int r = check_hip(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
   (void)hipGraphDestroy(g);
   if (r) return r;
-  // upload now, so that the first launch does not pay it
-  if ((r = check_hip(hipGraphUpload(ge, static_cast<hipStream_t>(stream)), "hipGraphUpload"))) {
-    (void)hipGraphExecDestroy(ge);
-    return r;
-  }
   *graph_exec = ge;
   return 0;
 }

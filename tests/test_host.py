@@ -320,3 +320,17 @@ def test_force_field_without_guo_forcing_is_refused():
     sim, _, _, _ = cases.adjoint_channel(8, 6, steps=4, forcing=None)
     with pytest.raises(ValueError, match="Guo forcing"):
         FlowRollout(sim, 4)
+
+
+def test_bench_child_leg_returns_record_or_error():
+    """The benchmark runs its adjoint leg in a child process: the child's
+    record comes back, a child that aborts gives an error record (and the
+    parent carries on), and an unused child exits when the pipe closes."""
+    import bench
+    assert bench.ChildLeg("tests.bench_child_probe:ok").run(timeout=120) == \
+        {"ok": 1}
+    rec = bench.ChildLeg("tests.bench_child_probe:abort").run(timeout=120)
+    assert "error" in rec and "status -6" in rec["error"], rec
+    leg = bench.ChildLeg("tests.bench_child_probe:ok")
+    leg.close()
+    assert leg.proc.returncode == 0
