@@ -336,12 +336,21 @@ def mrt_collide(case, stress_rates=None):
     With unit rates on rows 0..6 only the stress rows 7 (p_xx) and 8 (p_xy)
     keep a part of f - feq, and M^-1 = M^T diag(1/|row|^2) with
     |row 7|^2 = |row 8|^2 = 4, so the operator of ``mrt_collide_general``
-    becomes, in this operation order:
+    becomes f*_k = feq_k + 0.5 Fbar_k + sigma_k (1 - s) / 4 (p + 0.5 p^F),
+    with p the stress moment of f - feq and p^F that of Fbar.  The stress
+    moments of feq and Fbar have closed forms (sums over the equilibrium
+    and source formulas, with the lattice's own w and inv_cs_4):
+      p_xx(feq) = w1 ics4 rho (ux^2 - uy^2),  p_xy(feq) = 4 w5 ics4 rho ux uy,
+      p_xx(Fbar) = 2 w1 ics4 (ux fx - uy fy),
+      p_xy(Fbar) = 4 w5 ics4 (ux fy + uy fx)   (second-order Guo),
+      p(Fbar) = 0                               (linear Guo, no forcing).
+    In this operation order:
 
       q_k = feq_k + 0.5 Fbar_k            (feq_k alone without forcing)
-      e_k = f_k - feq_k
-      a7 = (e1 + e3) - (e2 + e4) + 0.5 ((F1 + F3) - (F2 + F4))
-      a8 = (e5 + e7) - (e6 + e8) + 0.5 ((F5 + F7) - (F6 + F8))
+      a7 = ((f1 + f3) - (f2 + f4)) - ((w1 ics4) rho) (ux ux - uy uy)
+      a8 = ((f5 + f7) - (f6 + f8)) - (((4 w5) ics4) rho) (ux uy)
+        second order:  a7 = a7 + (w1 ics4) (ux fx - uy fy)
+                       a8 = a8 + ((2 w5) ics4) (ux fy + uy fx)
       r7 = ((1 - s7) * 0.25) * a7,  r8 = ((1 - s8) * 0.25) * a8
       f*_0 = q_0;  f*_k = q_k + r7 (k = 1, 3), - r7 (2, 4),
                           + r8 (5, 7), - r8 (6, 8)
@@ -354,15 +363,18 @@ def mrt_collide(case, stress_rates=None):
     ux, uy = case.velocity[m_, 0], case.velocity[m_, 1]
     fx, fy = case.force_field[m_, 0], case.force_field[m_, 1]
     pn = case.pop_new[m_]
+    w1, w5, ics4 = c["w"][1], c["w"][5], c["inv_cs_4"]
     q = [equilibrium(rho, ux, uy, k, c) for k in range(9)]
-    e = [pn[:, k] - q[k] for k in range(9)]
-    a7 = (e[1] + e[3]) - (e[2] + e[4])
-    a8 = (e[5] + e[7]) - (e[6] + e[8])
+    a7 = ((pn[:, 1] + pn[:, 3]) - (pn[:, 2] + pn[:, 4])) - \
+        ((w1 * ics4) * rho) * (ux * ux - uy * uy)
+    a8 = ((pn[:, 5] + pn[:, 7]) - (pn[:, 6] + pn[:, 8])) - \
+        (((4.0 * w5) * ics4) * rho) * (ux * uy)
     if case.forcing is not None:
+        if case.forcing == "guo_second_order":
+            a7 = a7 + (w1 * ics4) * (ux * fx - uy * fy)
+            a8 = a8 + ((2.0 * w5) * ics4) * (ux * fy + uy * fx)
         fb = [guo_source(k, ux, uy, fx, fy, case.forcing, c)
               for k in range(9)]
-        a7 = a7 + 0.5 * ((fb[1] + fb[3]) - (fb[2] + fb[4]))
-        a8 = a8 + 0.5 * ((fb[5] + fb[7]) - (fb[6] + fb[8]))
         q = [q[k] + 0.5 * fb[k] for k in range(9)]
     s7, s8 = (om, om) if stress_rates is None else stress_rates
     r7 = ((1.0 - np.float64(s7)) * 0.25) * a7

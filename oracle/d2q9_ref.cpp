@@ -79,24 +79,23 @@ static inline double guo_k(const oracle_case* c, int k, double ux, double uy,
 
 // Build-defined MRT with Guo forcing and S = (1,1,1,1,1,1,1,omega,omega),
 // reduced form of d2q9_oracle.mrt_collide, same operation order:
-// q = feq (+ 0.5 Fbar), stress moments a7, a8 of f - feq (+ half of
-// Fbar's), r = ((1 - omega) * 0.25) a, f* = q +- r.
+// q = feq (+ 0.5 Fbar), stress moments a7, a8 of f minus the closed-form
+// stress of feq (+ half the closed-form stress of a second-order Fbar),
+// r = ((1 - omega) * 0.25) a, f* = q +- r.
 static void mrt_collide(const oracle_case* c, const double* pn, double* po, double rho,
                         double ux, double uy, double fx, double fy) {
   const double om = c->omega;
-  double q[9], e[9];
-  for (int k = 0; k < 9; ++k) {
-    q[k] = feq_k(c, k, rho, ux, uy);
-    e[k] = pn[k] - q[k];
-  }
-  double a7 = (e[1] + e[3]) - (e[2] + e[4]);
-  double a8 = (e[5] + e[7]) - (e[6] + e[8]);
+  const double w1 = c->w[1], w5 = c->w[5], ics4 = c->inv_cs_4;
+  double q[9];
+  for (int k = 0; k < 9; ++k) q[k] = feq_k(c, k, rho, ux, uy);
+  double a7 = ((pn[1] + pn[3]) - (pn[2] + pn[4])) - ((w1 * ics4) * rho) * (ux * ux - uy * uy);
+  double a8 = ((pn[5] + pn[7]) - (pn[6] + pn[8])) - (((4.0 * w5) * ics4) * rho) * (ux * uy);
   if (c->forcing != 0) {
-    double fb[9];
-    for (int k = 0; k < 9; ++k) fb[k] = guo_k(c, k, ux, uy, fx, fy);
-    a7 = a7 + 0.5 * ((fb[1] + fb[3]) - (fb[2] + fb[4]));
-    a8 = a8 + 0.5 * ((fb[5] + fb[7]) - (fb[6] + fb[8]));
-    for (int k = 0; k < 9; ++k) q[k] = q[k] + 0.5 * fb[k];
+    if (c->forcing == 2) {
+      a7 = a7 + (w1 * ics4) * (ux * fx - uy * fy);
+      a8 = a8 + ((2.0 * w5) * ics4) * (ux * fy + uy * fx);
+    }
+    for (int k = 0; k < 9; ++k) q[k] = q[k] + 0.5 * guo_k(c, k, ux, uy, fx, fy);
   }
   const double cs = (1.0 - om) * 0.25;
   const double r7 = cs * a7, r8 = cs * a8;

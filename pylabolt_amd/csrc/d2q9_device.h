@@ -609,52 +609,37 @@ __device__ __forceinline__ void collide_mrt_pairs(float (&g)[9], const Consts<fl
 }
 
 // MRT, FP64 storage, in the operation order of the oracle's mrt_collide
-// (the reduced form above, direction by direction):
-//   q_k = feq_k (+ 0.5 Fbar_k),  e_k = f_k - feq_k,
-//   a7 = (e1 + e3) - (e2 + e4) (+ 0.5 ((F1 + F3) - (F2 + F4))),
-//   a8 = (e5 + e7) - (e6 + e8) (+ 0.5 ((F5 + F7) - (F6 + F8))),
+// (the reduced form above, with the stress moments of feq and Fbar in
+// closed form):
+//   q_k = feq_k (+ 0.5 Fbar_k),
+//   a7 = ((f1 + f3) - (f2 + f4)) - ((w1 ics4) rho) (ux ux - uy uy)
+//        (+ (w1 ics4) (ux fx - uy fy) with second-order Guo),
+//   a8 = ((f5 + f7) - (f6 + f8)) - (((4 w5) ics4) rho) (ux uy)
+//        (+ ((2 w5) ics4) (ux fy + uy fx) with second-order Guo),
 //   r = ((1 - s) * 0.25) a,  f*_k = q_k +- r (f*_0 = q_0).
+// The populations are consumed by the two stress sums; each output is then
+// one equilibrium (and source) evaluation, so nothing of the nine is held.
 template <typename T, int FORCING, bool DEV>
 __device__ __forceinline__ void collide_mrt(T (&g)[9], const Consts<T>& K,
                                             T drho, T rho, T ux, T uy, T fx, T fy) {
   static_assert(!DEV, "deviation storage forms use collide_mrt_pairs");
   constexpr bool forced = FORCING != D2Q9_FORCING_NONE;
   const T ics2 = K.ics2, ics4 = K.ics4;
-  // The stress moments of f - feq first, consuming g; then the source, and
-  // the equilibrium formed again for the result from copies of rho and u
-  // the compiler cannot see through (so that it cannot keep the first
-  // evaluation's nine values live instead).  The same bits as holding
-  // them; about a quarter more FP64 operations, but the two-step kernel stays
-  // at 64 VGPRs without spilling (4096x1024 MRT + second-order Guo
-  // channel: 71.8 us per step against 82.5 us holding them).
-  T a7, a8;
-  {
-    const T u2 = ux * ux + uy * uy;
-    T e[9];
-#pragma unroll
-    for (int k = 1; k < 9; ++k) e[k] = g[k] - feq_k<T>(k, rho, ux, uy, u2, K.w[k], ics2, ics4);
-    a7 = (e[1] + e[3]) - (e[2] + e[4]);
-    a8 = (e[5] + e[7]) - (e[6] + e[8]);
-  }
-  T hf[9];
-  if constexpr (forced) {
-    T fb[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) fb[k] = guo_k<T, FORCING>(k, ux, uy, fx, fy, K.w[k], ics2, ics4);
-    a7 = a7 + (T)0.5 * ((fb[1] + fb[3]) - (fb[2] + fb[4]));
-    a8 = a8 + (T)0.5 * ((fb[5] + fb[7]) - (fb[6] + fb[8]));
-#pragma unroll
-    for (int k = 0; k < 9; ++k) hf[k] = (T)0.5 * fb[k];
+  const T w1 = K.w[1], w5 = K.w[5];
+  T a7 = ((g[1] + g[3]) - (g[2] + g[4])) - ((w1 * ics4) * rho) * (ux * ux - uy * uy);
+  T a8 = ((g[5] + g[7]) - (g[6] + g[8])) - ((((T)4 * w5) * ics4) * rho) * (ux * uy);
+  if constexpr (FORCING == D2Q9_GUO_SECOND_ORDER) {
+    a7 = a7 + (w1 * ics4) * (ux * fx - uy * fy);
+    a8 = a8 + (((T)2 * w5) * ics4) * (ux * fy + uy * fx);
   }
   const T r7 = (((T)1 - K.rates[7]) * (T)0.25) * a7;
   const T r8 = (((T)1 - K.rates[8]) * (T)0.25) * a8;
-  T r2 = rho, x2 = ux, y2 = uy;
-  asm volatile("" : "+v"(r2), "+v"(x2), "+v"(y2) : "v"(r7), "v"(r8));
-  const T u2b = x2 * x2 + y2 * y2;
+  const T u2 = ux * ux + uy * uy;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    T q = feq_k<T>(k, r2, x2, y2, u2b, K.w[k], ics2, ics4);
-    if constexpr (forced) q = q + hf[k];
+    T q = feq_k<T>(k, rho, ux, uy, u2, K.w[k], ics2, ics4);
+    if constexpr (forced)
+      q = q + (T)0.5 * guo_k<T, FORCING>(k, ux, uy, fx, fy, K.w[k], ics2, ics4);
     if (k == 0) g[0] = q;
     else if (k <= 4) g[k] = (k & 1) ? q + r7 : q - r7;
     else g[k] = (k & 1) ? q + r8 : q - r8;
