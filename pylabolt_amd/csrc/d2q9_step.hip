@@ -179,9 +179,21 @@ __device__ __forceinline__ void deep_wall_global(const d2q9_lattice& L,
 }
 
 // Two workgroups (32 waves) per CU need 8 waves per SIMD, i.e. at most 64
-// VGPRs, and every form is held to that: BGK takes 43-62 VGPRs, the
-// reduced MRT 40-64 (FP64 MRT with second-order Guo at 64, with two spill
-// instructions).
+// VGPRs, and every form is held to that: BGK takes 42-62 VGPRs, the
+// reduced MRT 40-56.
+//
+// Whether the wave index is taken as wave-uniform (readfirstlane), so that
+// the rows' addresses are formed once per wave in scalar registers instead
+// of per lane: that removes 13-20 % of the deep kernel's vector
+// instructions, and measured (same box, identical results) FP64 BGK
+// cylinder 8192x2048 -0.7 %, FP16 -1.5 %, the 4096x1024 MRT + Guo channel
+// -0.5 % (FP64) and -3 % (FP16, BF16), but FP32 BGK +1.1 % and BF16 BGK
+// +2.6 %; those two keep the per-lane form.
+template <int S, int MODEL>
+constexpr bool deep_scalar_rows() {
+  return !(MODEL == D2Q9_BGK && (S == D2Q9_STORE_F32 || S == D2Q9_STORE_BF16DEV));
+}
+
 template <int S, int MODEL, int FORCING>
 __global__ __launch_bounds__(64 * DEEP_WAVES, D2Q9_DEEP_WAVES_PER_SIMD) void k_deep2(
     const d2q9_lattice L, const int32_t* __restrict__ deep, const uint8_t* __restrict__ walls,
@@ -194,7 +206,9 @@ __global__ __launch_bounds__(64 * DEEP_WAVES, D2Q9_DEEP_WAVES_PER_SIMD) void k_d
   constexpr int CPL = C::CPL;
   constexpr int B = 64 * CPL;
   __shared__ T s1[DEEP_R1][9][B + 2];
-  const int lane = (int)threadIdx.x, w = (int)threadIdx.y;
+  const int lane = (int)threadIdx.x;
+  const int w = deep_scalar_rows<S, MODEL>() ? __builtin_amdgcn_readfirstlane((int)threadIdx.y)
+                                             : (int)threadIdx.y;
   const int p = xcd_remap((int)blockIdx.x, n_deep);
   const int t = deep[p];
   const int wf = walls ? (int)walls[p] : 0;
