@@ -165,7 +165,7 @@ __device__ __forceinline__ void deep_collide(const d2q9_lattice& L,
 // value its inner neighbour pulled (which lies at the wall cell's own
 // column) and, for bounce-back, the cell's own post-collision value of the
 // opposite direction: the rules of the bulk sweep's wall rows.
-template <int S>
+template <int S, bool WRAP>
 __device__ __forceinline__ void deep_wall_global(const d2q9_lattice& L,
                                                  const Consts<typename Codec<S>::T>& K,
                                                  const typename Codec<S>::raw* __restrict__ fsrc,
@@ -174,7 +174,8 @@ __device__ __forceinline__ void deep_wall_global(const d2q9_lattice& L,
 #pragma unroll
   for (int d = 1; d < 9; ++d) {
     if (CY[d] != cy) continue;
-    g[d] = kind == 1 ? ld<S>(L, K, fsrc, i - CX[d], j, d) : ld<S>(L, K, fsrc, i, j, INV[d]);
+    g[d] = kind == 1 ? ld<S>(L, K, fsrc, WRAP ? wrap_row(L, i - CX[d]) : i - CX[d], j, d)
+                     : ld<S>(L, K, fsrc, WRAP ? wrap_row(L, i) : i, j, INV[d]);
   }
 }
 
@@ -194,7 +195,64 @@ constexpr bool deep_scalar_rows() {
   return !(MODEL == D2Q9_BGK && (S == D2Q9_STORE_F32 || S == D2Q9_STORE_BF16DEV));
 }
 
-template <int S, int MODEL, int FORCING>
+// Stage one of k_deep2 (see there): the grown rows (waves 0 .. DEEP_R1 - 1)
+// and the two grown columns (the last two waves) of step n + 1, into LDS.
+// WRAP: the tile reaches past the first or last row of a lattice periodic
+// in x, whose rows are then read from their periodic images; only such
+// tiles pay for the wrap.
+template <int S, int MODEL, int FORCING, bool WRAP>
+__device__ __forceinline__ void deep_stage1(
+    const d2q9_lattice& L, const Consts<typename Codec<S>::T>& K,
+    const typename Codec<S>::raw* __restrict__ fsrc,
+    typename Codec<S>::T (&s1)[DEEP_R1][9][64 * Codec<S>::CPL + 2], int w, int lane, int i0,
+    int j0, int tj, int lo, int hi) {
+  using T = typename Codec<S>::T;
+  constexpr int CPL = Codec<S>::CPL;
+  constexpr int B = 64 * CPL;
+  if (w < DEEP_R1) {
+    // grown row i0 - 1 + w, the block's B cells; the loads issue at raised
+    // wave priority, so that they reach memory ahead of other waves'
+    // arithmetic (measured: 0.7 % on the FP64 step)
+    const int i = i0 - 1 + w, jl = j0 + lane * CPL;
+    T g[CPL][9];
+    __builtin_amdgcn_s_setprio(2);
+    RawBlock<S> rb;
+    gather_issue<S, PHASE_FUSED, WRAP>(L, fsrc, i, tj, lane, jl, rb);
+    __builtin_amdgcn_s_setprio(0);
+    gather_finish<S, PHASE_FUSED>(K, rb, lane, g);
+    if (lo && lane == 0) deep_wall_global<S, WRAP>(L, K, fsrc, g[0], lo, i, 1, 1);
+    if (hi && lane == 63) deep_wall_global<S, WRAP>(L, K, fsrc, g[CPL - 1], hi, i, L.ny, -1);
+    const int64_t c = cell_index(L, WRAP ? wrap_row(L, i) : i, jl);
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      T rho, ux, uy;
+      deep_collide<S, MODEL, FORCING>(L, K, g[q], c + q, rho, ux, uy);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) s1[w][k][1 + lane * CPL + q] = g[q][k];
+    }
+  } else if (lane < DEEP_R1 && !((w - DEEP_R1) ? hi : lo)) {
+    // grown column j0 - 1 (wave DEEP_R1) or j0 + B (wave DEEP_R1 + 1), not
+    // the ghost column beside a wall
+    const int r = lane, cl = (w - DEEP_R1) ? B + 1 : 0;
+    const int i = i0 - 1 + r, j = j0 - 1 + cl;
+    T g[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      g[k] = ld<S>(L, K, fsrc, WRAP ? wrap_row(L, i - CX[k]) : i - CX[k], j - CY[k], k);
+    T rho, ux, uy;
+    deep_collide<S, MODEL, FORCING>(L, K, g, cell_index(L, WRAP ? wrap_row(L, i) : i, j), rho,
+                                    ux, uy);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s1[r][k][cl] = g[k];
+  }
+}
+
+// XW: the lattice is periodic in x; the kernel then holds the stage-one
+// variant that reads rows past the ends from their images, taken by the
+// tiles next to them.  (Compiled into the kernel of every lattice, the
+// second variant had slowed the FP16 cylinder 8192x2048 step by 3 %, though
+// none of its tiles takes it: a kernel of its own for periodic lattices.)
+template <int S, int MODEL, int FORCING, bool XW>
 __global__ __launch_bounds__(64 * DEEP_WAVES, D2Q9_DEEP_WAVES_PER_SIMD) void k_deep2(
     const d2q9_lattice L, const int32_t* __restrict__ deep, const uint8_t* __restrict__ walls,
     int n_deep, int ntj, const typename Codec<S>::raw* __restrict__ fsrc,
@@ -214,14 +272,36 @@ __global__ __launch_bounds__(64 * DEEP_WAVES, D2Q9_DEEP_WAVES_PER_SIMD) void k_d
   const int wf = walls ? (int)walls[p] : 0;
   const int lo = wf & 3, hi = (wf >> 2) & 3;
   const int tj = t % ntj;
-  const int i0 = 1 + (t / ntj) * DEEP_ROWS, j0 = 1 + tj * B;
+  // in a lattice periodic in x, the last row of tiles of a lattice whose
+  // rows are not a multiple of DEEP_ROWS ends at row nx, overlapping the
+  // row of tiles before it (cells swept by two tiles are written twice with
+  // the same bits)
+  const int i0 = XW ? min(1 + (t / ntj) * DEEP_ROWS, L.nx - DEEP_ROWS + 1)
+                    : 1 + (t / ntj) * DEEP_ROWS;
+  const int j0 = 1 + tj * B;
   // the whole workgroup leaves (before its barrier) for a tile whose grown
-  // region is not inside the lattice, but for the ghost column beside a
-  // wall: every access then stays in bounds
-  if (t < 0 || i0 - 1 < 1 || i0 + DEEP_ROWS > L.nx || (j0 - 1 < 1 && !(lo && j0 == 1)) ||
-      (j0 + B > L.ny && !(hi && j0 + B - 1 == L.ny)))
+  // region is not inside the lattice -- but for the ghost column beside a
+  // wall, and for rows past the first and last of a lattice periodic in x
+  // (read from their periodic images; at least 2 DEEP_ROWS rows): every
+  // access then stays in bounds
+  if (t < 0 ||
+      (XW ? (!L.x_periodic || L.nx < 2 * DEEP_ROWS || i0 < 1)
+          : (i0 - 1 < 1 || i0 + DEEP_ROWS > L.nx)) ||
+      (j0 - 1 < 1 && !(lo && j0 == 1)) || (j0 + B > L.ny && !(hi && j0 + B - 1 == L.ny)))
     return;
   const Consts<T> K = load_consts<T>(L);
+  // rows past the lattice's ends are read only by a tile of a lattice
+  // periodic in x whose pulls reach row 0 or nx + 1
+  if constexpr (XW) {
+    if (i0 < 3 || i0 + DEEP_ROWS + 1 > L.nx)
+      deep_stage1<S, MODEL, FORCING, true>(L, K, fsrc, s1, w, lane, i0, j0, tj, lo, hi);
+    else
+      deep_stage1<S, MODEL, FORCING, false>(L, K, fsrc, s1, w, lane, i0, j0, tj, lo, hi);
+  } else {
+  // The same statements as deep_stage1<..., false>, kept inline: the
+  // compiler then schedules this kernel exactly as it did before the
+  // periodic variant existed (through the function, the FP16 cylinder
+  // 8192x2048 step measured 2.4 % slower).
   if (w < DEEP_R1) {
     // grown row i0 - 1 + w, the block's B cells; the loads issue at raised
     // wave priority, so that they reach memory ahead of other waves'
@@ -233,8 +313,8 @@ __global__ __launch_bounds__(64 * DEEP_WAVES, D2Q9_DEEP_WAVES_PER_SIMD) void k_d
     gather_issue<S, PHASE_FUSED>(L, fsrc, i, tj, lane, jl, rb);
     __builtin_amdgcn_s_setprio(0);
     gather_finish<S, PHASE_FUSED>(K, rb, lane, g);
-    if (lo && lane == 0) deep_wall_global<S>(L, K, fsrc, g[0], lo, i, 1, 1);
-    if (hi && lane == 63) deep_wall_global<S>(L, K, fsrc, g[CPL - 1], hi, i, L.ny, -1);
+    if (lo && lane == 0) deep_wall_global<S, false>(L, K, fsrc, g[0], lo, i, 1, 1);
+    if (hi && lane == 63) deep_wall_global<S, false>(L, K, fsrc, g[CPL - 1], hi, i, L.ny, -1);
     const int64_t c = cell_index(L, i, jl);
 #pragma unroll
     for (int q = 0; q < CPL; ++q) {
@@ -250,11 +330,14 @@ __global__ __launch_bounds__(64 * DEEP_WAVES, D2Q9_DEEP_WAVES_PER_SIMD) void k_d
     const int i = i0 - 1 + r, j = j0 - 1 + cl;
     T g[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) g[k] = ld<S>(L, K, fsrc, i - CX[k], j - CY[k], k);
+    for (int k = 0; k < 9; ++k)
+      g[k] = ld<S>(L, K, fsrc, i - CX[k], j - CY[k], k);
     T rho, ux, uy;
-    deep_collide<S, MODEL, FORCING>(L, K, g, cell_index(L, i, j), rho, ux, uy);
+    deep_collide<S, MODEL, FORCING>(L, K, g, cell_index(L, i, j), rho,
+                                    ux, uy);
 #pragma unroll
     for (int k = 0; k < 9; ++k) s1[r][k][cl] = g[k];
+  }
   }
   __syncthreads();
   if (w >= 1 && w <= DEEP_ROWS) {
@@ -695,20 +778,32 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
   if ((int64_t)n_deep > (int64_t)ntj * ((L->nx + DEEP_ROWS - 1) / DEEP_ROWS))
     return fail(D2Q9_E_ARG, "more deep tiles than the lattice holds");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  // with no rim (every cell in a deep tile: a channel periodic in x with
+  // straight walls at y) the step is the deep kernel alone on the caller's
+  // stream, with no fork and join
+  const bool rim_empty = L_rim_a->n_bulk + L_rim_a->n_general + L_rim_a->n_fix == 0 &&
+                         L_rim_b->n_bulk + L_rim_b->n_general + L_rim_b->n_fix == 0;
   Aux* a = nullptr;
-  if (int r = aux_for_stream(s, &a)) return r;
-  std::lock_guard<std::mutex> order(a->order);
-  if (int r = check_hip(hipEventRecord(a->fork, s), "hipEventRecord fork")) return r;
+  std::unique_lock<std::mutex> order;
+  if (!rim_empty) {
+    if (int r = aux_for_stream(s, &a)) return r;
+    order = std::unique_lock<std::mutex>(a->order);
+    if (int r = check_hip(hipEventRecord(a->fork, s), "hipEventRecord fork")) return r;
+  }
   if (n_deep > 0) {
     const dim3 grid((unsigned)n_deep), block(64, DEEP_WAVES);
     const int key = L->collision * 3 + L->forcing;
-#define D2Q9_DEEP(SV, KEY, M, F)                                                         \
-  case KEY:                                                                              \
-    hipLaunchKernelGGL((k_deep2<SV, M, F>), grid, block, 0, s, *L, deep_tiles, deep_walls,   \
-                       n_deep, ntj,                                                      \
-                       static_cast<const Codec<SV>::raw*>(f_src),                        \
-                       static_cast<Codec<SV>::raw*>(f_dst),                              \
-                       static_cast<Codec<SV>::T*>(mom_dst), write_moments);              \
+#define D2Q9_DEEP_XW(SV, M, F, XW)                                                       \
+  hipLaunchKernelGGL((k_deep2<SV, M, F, XW>), grid, block, 0, s, *L, deep_tiles, deep_walls, \
+                     n_deep, ntj, static_cast<const Codec<SV>::raw*>(f_src),             \
+                     static_cast<Codec<SV>::raw*>(f_dst),                                \
+                     static_cast<Codec<SV>::T*>(mom_dst), write_moments)
+#define D2Q9_DEEP(SV, KEY, M, F)                  \
+  case KEY:                                       \
+    if (L->x_periodic)                            \
+      D2Q9_DEEP_XW(SV, M, F, true);               \
+    else                                          \
+      D2Q9_DEEP_XW(SV, M, F, false);              \
     break;
 #define D2Q9_DEEP_ALL(SV)                                          \
   switch (key) {                                                   \
@@ -731,8 +826,10 @@ int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
     }
 #undef D2Q9_DEEP_ALL
 #undef D2Q9_DEEP
+#undef D2Q9_DEEP_XW
     if (int r = check_hip(hipGetLastError(), "k_deep2 launch")) return r;
   }
+  if (rim_empty) return 0;
   if (int r = check_hip(hipStreamWaitEvent(a->stream, a->fork, 0), "hipStreamWaitEvent fork"))
     return r;
   // rim: step n + 1 of the rim tiles and the cells they pull from into

@@ -97,7 +97,20 @@ struct RawBlock {
   typename Codec<S>::raw halo;
 };
 
-template <int S, int PHASE>
+// Row r of a lattice periodic in x, for r in [1 - nx, 2 nx]: the image in
+// 1..nx (the two-step sweep's grown rows reach past the first and last row).
+__device__ __forceinline__ int wrap_row(const d2q9_lattice& L, int r) {
+  if (L.x_periodic) {
+    if (r < 1) r += L.nx;
+    else if (r > L.nx) r -= L.nx;
+  }
+  return r;
+}
+
+// WRAP: rows outside 1..nx are taken from their periodic image (the
+// two-step sweep on a lattice periodic in x); otherwise row i - c_x is read
+// as it is (0 and nx + 1 being ghost rows).
+template <int S, int PHASE, bool WRAP = false>
 __device__ __forceinline__ void gather_issue(const d2q9_lattice& L,
                                              const typename Codec<S>::raw* __restrict__ fsrc,
                                              int i, int tj, int lane, int j0, RawBlock<S>& rb) {
@@ -105,10 +118,11 @@ __device__ __forceinline__ void gather_issue(const d2q9_lattice& L,
   using R = typename C::raw;
   constexpr int CPL = C::CPL;
   constexpr int B = 64 * CPL;
+  auto rw = [&](int r) { return WRAP ? wrap_row(L, r) : r; };
   if constexpr (CPL == 1) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      const int row = (PHASE == PHASE_COLLIDE) ? i : i - CX[k];
+      const int row = rw((PHASE == PHASE_COLLIDE) ? i : i - CX[k]);
       const int cy = (PHASE == PHASE_COLLIDE) ? 0 : CY[k];
       const int jc = (j0 <= L.ny) ? j0 - cy : j0;
       rb.w[k] = *reinterpret_cast<const typename RawBlock<S>::W*>(
@@ -117,7 +131,7 @@ __device__ __forceinline__ void gather_issue(const d2q9_lattice& L,
   } else {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      const int row = (PHASE == PHASE_COLLIDE) ? i : i - CX[k];
+      const int row = rw((PHASE == PHASE_COLLIDE) ? i : i - CX[k]);
       rb.w[k] = *reinterpret_cast<const typename RawBlock<S>::W*>(
           fsrc + (((int64_t)row * L.nbj + tj + 1) * 9 + k) * B + lane * CPL);
     }
@@ -130,7 +144,7 @@ __device__ __forceinline__ void gather_issue(const d2q9_lattice& L,
       const int jf = 1 + tj * B;             // first cell of the block
       int jh = h < 3 ? jf - 1 : jf + B;      // element before / after
       if (jh > L.ny + 1) jh = jf;            // past the padded row: unused
-      rb.halo = fsrc[pop_index<B>(L, i - cxh, jh, kh)];
+      rb.halo = fsrc[pop_index<B>(L, rw(i - cxh), jh, kh)];
     }
   }
 }

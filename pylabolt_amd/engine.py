@@ -223,26 +223,36 @@ class CaseTables:
         # for zero-gradient, the source cell must be a plain pull of the
         # same block and not itself such a cell
         side = self.wall_side_pad[1:-1, 1:-1]
-        edge = np.zeros_like(fluid)
+
+        def inline_walls(edge):
+            plain_bc = (inner == (nt.CELL_FLUID | nt.CELL_BC)) & ~edge
+            cand = plain_bc & (side >= 0) & (side <= 1)
+            zg = plain_bc & (side >= 2)
+            for s_, step in ((0, 1), (1, -1)):
+                ci, cj = np.nonzero(cand & (side == s_))
+                sj = cj + step
+                ok = (sj >= 0) & (sj < ny)
+                ci, cj, sj = ci[ok], cj[ok], sj[ok]
+                src_flags = inner[ci, sj]
+                ok = (((src_flags & nt.CELL_FLUID) != 0) &
+                      ((src_flags & nt.CELL_NEAR_SOLID) == 0) &
+                      ~edge[ci, sj] & ~cand[ci, sj] &
+                      (cj // self.tile_j == sj // self.tile_j))
+                zg[ci[ok], cj[ok]] = True
+            return zg
+
+        edge_y = np.zeros_like(fluid)
+        if self.y_periodic:
+            edge_y[:, 0] = edge_y[:, -1] = True
+        edge = edge_y.copy()
         if self.x_periodic:
             edge[0, :] = edge[-1, :] = True
-        if self.y_periodic:
-            edge[:, 0] = edge[:, -1] = True
-        plain_bc = (inner == (nt.CELL_FLUID | nt.CELL_BC)) & ~edge
-        cand = plain_bc & (side >= 0) & (side <= 1)
-        zg = plain_bc & (side >= 2)
-        for s_, step in ((0, 1), (1, -1)):
-            ci, cj = np.nonzero(cand & (side == s_))
-            sj = cj + step
-            ok = (sj >= 0) & (sj < ny)
-            ci, cj, sj = ci[ok], cj[ok], sj[ok]
-            src_flags = inner[ci, sj]
-            ok = (((src_flags & nt.CELL_FLUID) != 0) &
-                  ((src_flags & nt.CELL_NEAR_SOLID) == 0) &
-                  ~edge[ci, sj] & ~cand[ci, sj] &
-                  (cj // self.tile_j == sj // self.tile_j))
-            zg[ci[ok], cj[ok]] = True
+        zg = inline_walls(edge)
         self.wall_inline = zg
+        # the two-step sweep reads the rows past a periodic lattice's ends
+        # from their images, so there the wall cells of the first and last
+        # rows are straight-wall cells too
+        self.wall_inline_wrap = inline_walls(edge_y) if self.x_periodic else zg
         special &= ~zg
         ntj = -(-ny // self.tile_j)
         nti = -(-nx // self.tile_i)
@@ -345,19 +355,34 @@ class CaseTables:
         nx, ny = self.nx, self.ny
         inner = self.flags_pad[1:-1, 1:-1]
         plain = inner == nt.CELL_FLUID
-        if self.x_periodic:
+        # In a lattice periodic in x with at least two tile heights of rows,
+        # the deep kernel reads the rows past the first and last from their
+        # periodic images, so the edge rows are plain; and when the rows are
+        # a multiple of the tile rows, the last row of deep tiles is moved to
+        # end at the last row (overlapping the one before it), as the kernel
+        # does for a tile that would pass it.  A channel periodic in x with
+        # straight walls at y is then deep throughout: no rim.
+        wrap = bool(self.x_periodic) and nx >= 2 * rows
+        if self.x_periodic and not wrap:
             plain[0, :] = plain[-1, :] = False
         if self.y_periodic:
             plain[:, 0] = plain[:, -1] = False
-        sat = np.zeros((nx + 1, ny + 1), dtype=np.int64)
-        sat[1:, 1:] = (~plain).astype(np.int64).cumsum(0).cumsum(1)
+        ext = (lambda a: np.concatenate([a[-1:], a, a[:1]], axis=0)) if wrap \
+            else (lambda a: a)
+        off = 1 if wrap else 0          # row r (0-based) is row r + off of ext(...)
+        plain_x = ext(plain)
+        sat = np.zeros((plain_x.shape[0] + 1, ny + 1), dtype=np.int64)
+        sat[1:, 1:] = (~plain_x).astype(np.int64).cumsum(0).cumsum(1)
         ntj = -(-ny // B)
         nti2 = -(-nx // rows)
-        ti = np.arange(nti2)[:, None]
+        first = np.arange(nti2) * rows  # first row of each row of tiles
+        if wrap and nx % self.tile_i == 0:
+            first = np.minimum(first, nx - rows)
+        ti = first[:, None]
         tj = np.arange(ntj)[None, :]
-        # grown region (0-based, inclusive): rows ti*rows - 1 .. ti*rows +
-        # rows, cells tj*B - 1 .. tj*B + B
-        r0, r1 = ti * rows - 1, ti * rows + rows
+        # grown region (0-based, inclusive): rows first - 1 .. first + rows,
+        # cells tj*B - 1 .. tj*B + B
+        r0, r1 = ti - 1, ti + rows
         c0, c1 = tj * B - 1, tj * B + B
         # a tile at the low (high) wall: its grown region reaches the ghost
         # column, and the wall column is checked apart from the plain cells
@@ -365,9 +390,10 @@ class CaseTables:
         at_hi = (c1 == ny) & np.ones_like(r0, dtype=bool)
         ca = np.where(at_lo, 1, c0)
         cb = np.where(at_hi, ny - 2, c1)
-        ok = ((r0 >= 0) & (r1 <= nx - 1) & (ca >= 0) & (cb <= ny - 1) &
-              (ca <= cb))
-        r0c, r1c = np.clip(r0, 0, nx - 1), np.clip(r1, 0, nx - 1)
+        rows_ok = (r1 - 1 <= nx - 1) if wrap else ((r0 >= 0) & (r1 <= nx - 1))
+        ok = rows_ok & (ca >= 0) & (cb <= ny - 1) & (ca <= cb)
+        nxe = plain_x.shape[0]
+        r0c, r1c = np.clip(r0 + off, 0, nxe - 1), np.clip(r1 + off, 0, nxe - 1)
         cac, cbc = np.clip(ca, 0, ny - 1), np.clip(cb, 0, ny - 1)
         bad = (sat[r1c + 1, cbc + 1] - sat[r0c, cbc + 1] - sat[r1c + 1, cac]
                + sat[r0c, cac])
@@ -376,7 +402,7 @@ class CaseTables:
         # wall cell the bulk sweep does inline, sides 0/2 at j = 1, 1/3 at
         # j = ny)
         side = self.wall_side_pad[1:-1, 1:-1]
-        inl = self.wall_inline
+        inl = self.wall_inline_wrap if wrap else self.wall_inline
         lo_row = np.where(inl[:, 0] & (side[:, 0] == 0), 1,
                           np.where(inl[:, 0] & (side[:, 0] == 2), 2, 0))
         hi_row = np.where(inl[:, -1] & (side[:, -1] == 1), 1,
@@ -385,6 +411,7 @@ class CaseTables:
 
         def kind_over_rows(per_row):
             kind = np.zeros_like(r0)
+            per_row = ext(per_row)
             for k in (1, 2):
                 cum = np.concatenate([[0], np.cumsum(per_row == k)])
                 full = (cum[r1c + 1] - cum[r0c]) == n_rows
@@ -395,10 +422,12 @@ class CaseTables:
         hi_kind = np.where(at_hi, kind_over_rows(hi_row), 0)
         ok &= ~at_lo | (lo_kind > 0)
         ok &= ~at_hi | (hi_kind > 0)
-        deep_ids = (ti * ntj + tj)[ok].astype(np.int32)
+        tix = np.arange(nti2)[:, None]
+        deep_ids = (tix * ntj + tj)[ok].astype(np.int32)
         deep_walls = (lo_kind | (hi_kind << 2))[ok].astype(np.uint8)
-        covered = np.repeat(np.repeat(ok, rows, axis=0), B,
-                            axis=1)[:nx, :ny]
+        covered = np.zeros((nx, ny), dtype=bool)
+        for a in range(nti2):
+            covered[first[a]:first[a] + rows] |= np.repeat(ok[a], B)[:ny]
         # rim tiles of the current tile shape
         nti = self.nti
         cpad = np.zeros((nti * self.tile_i, ntj * B), dtype=bool)
@@ -419,6 +448,9 @@ class CaseTables:
             bulk_masks = np.zeros(1, dtype=np.uint64)
         wall_index = (self.wall_index[bulk_pos].astype(np.int32)
                       if self.wall_masks is not None else None)
+        if wall_index is not None and wall_index.size == 0:
+            # no rim bulk tile: a placeholder, so that the array is not null
+            wall_index = np.full(1, -1, dtype=np.int32)
         pitch = int(math.ceil((ny + 2) / ALIGN) * ALIGN)
 
         def cells_of(c):

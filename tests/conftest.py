@@ -7,6 +7,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# The C++ restatement runs OpenMP loops; with several test processes on the
+# same cores (pytest -n), spinning idle threads slowed its tests forty-fold.
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
+
 
 def pytest_configure(config):
     config.addinivalue_line(

@@ -43,6 +43,11 @@ CASES = {
     "cylinder_walls_both": lambda: cases.cylinder(100, 256, diameter=16,
                                                   xc=50),
     "channel_walls_both": lambda: _forced_channel(256),
+    # periodic rows read from their images and the last row of tiles moved
+    # up to end at the last row (80 = 6 x 12 + 8): every cell deep, no rim;
+    # and rows a multiple of the tile height
+    "channel_rows_multiple": lambda: cases.poiseuille_channel(
+        72, 256, model="BGK", forcing="guo_linear"),
 }
 
 
@@ -65,6 +70,8 @@ def test_two_step_matches_oracle(name):
     if name.endswith("walls_both"):
         walls = eng._deep["walls"].cpu().numpy()[:eng.n_deep]
         assert (walls & 3).any() and (walls >> 2).any()
+    if name in ("channel_walls_both", "channel_rows_multiple"):
+        assert eng.deep_fraction == 1.0
     print(f"[two-step] {name}: {eng.n_deep} deep tiles, "
           f"{eng.deep_fraction:.0%} of the cells", flush=True)
     done = 0

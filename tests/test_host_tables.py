@@ -183,16 +183,20 @@ DEEP_CASES = {
                                                   xc=50),
     "cavity": lambda: cases.lid_driven_cavity(100, 200),
     "channel": lambda: cases.poiseuille_channel(80, 256, model="BGK"),
+    "channel_rows_multiple": lambda: cases.poiseuille_channel(72, 192),
+    "channel_rows_odd": lambda: cases.poiseuille_channel(77, 130),
 }
 
 
 @pytest.mark.parametrize("name", sorted(DEEP_CASES))
 def test_deep_split_covers_and_feeds_the_rim(name):
-    """Deep tiles of the two-step sweep: block-aligned, disjoint, plain fluid
-    one cell around but for a flagged straight wall column (zero-gradient
-    or bounce-back wall cells swept inline); every other cell lies in a
-    listed rim tile; the first rim step (list A) computes every cell the
-    second (list B) pulls from."""
+    """Deep tiles of the two-step sweep: block-aligned, plain fluid one cell
+    around but for a flagged straight wall column (zero-gradient or
+    bounce-back wall cells swept inline), disjoint but for the last row of
+    tiles of a lattice periodic in x (moved up to end at its last row, as
+    the kernel moves it; rows past the lattice's ends are their periodic
+    images); every other cell lies in a listed rim tile; the first rim step
+    (list A) computes every cell the second (list B) pulls from."""
     from pylabolt_amd import _native as nt
     st = State(DEEP_CASES[name]())
     nx, ny = (int(v) for v in st.mesh.grid_global_shape)
@@ -205,36 +209,54 @@ def test_deep_split_covers_and_feeds_the_rim(name):
     inner = t.flags_pad[1:-1, 1:-1]
     fluid = (inner & nt.CELL_FLUID) != 0
     plain = inner == nt.CELL_FLUID
-    if t.x_periodic:
+    wrap = t.x_periodic and nx >= 2 * rows
+    if t.x_periodic and not wrap:
         plain[[0, -1], :] = False
     if t.y_periodic:
         plain[:, [0, -1]] = False
     ntj = -(-ny // 64)
     cov = np.zeros((nx, ny), dtype=int)
     side = t.wall_side_pad[1:-1, 1:-1]
+    inl = t.wall_inline_wrap if wrap else t.wall_inline
     for tid, wf in zip(d.deep_ids[:d.n_deep], d.deep_walls[:d.n_deep]):
         ti, tj = divmod(int(tid), ntj)
         lo, hi = int(wf) & 3, (int(wf) >> 2) & 3
         a, b = ti * rows, tj * 64
-        assert a + rows <= nx - 1 and a >= 1
+        if a + rows > nx:                    # moved up, as the kernel does
+            a = nx - rows
+            assert wrap and nx % 4 == 0
+        if wrap:
+            assert 0 <= a and a + rows <= nx
+        else:
+            assert a + rows <= nx - 1 and a >= 1
+        grown = np.arange(a - 1, a + rows + 1) % nx   # periodic images
         c0 = b - 1 if not lo else b + 1      # the wall column apart
         c1 = b + 64 if not hi else b + 62
         if lo:
             assert b == 0
             # the wall column: inline wall cells of the flagged kind
             want = 0 if lo == 1 else 2
-            assert t.wall_inline[a - 1:a + rows + 1, 0].all()
-            assert (side[a - 1:a + rows + 1, 0] == want).all()
+            assert inl[grown, 0].all()
+            assert (side[grown, 0] == want).all()
         if hi:
             assert b + 64 == ny
             want = 1 if hi == 1 else 3
-            assert t.wall_inline[a - 1:a + rows + 1, ny - 1].all()
-            assert (side[a - 1:a + rows + 1, ny - 1] == want).all()
+            assert inl[grown, ny - 1].all()
+            assert (side[grown, ny - 1] == want).all()
         assert c0 >= 0 and c1 <= ny - 1
-        assert plain[a - 1:a + rows + 1, c0:c1 + 1].all()
+        assert plain[grown][:, c0:c1 + 1].all()
         cov[a:a + rows, b:b + 64] += 1
-    assert cov.max() == 1
-    assert np.array_equal(cov == 1, d.covered)
+    assert cov.max() <= (2 if wrap else 1)
+    assert np.array_equal(cov >= 1, d.covered)
+    if wrap and nx % 4 == 0 and ny % 64 == 0 and name.startswith("channel"):
+        # a channel periodic in x with straight walls at y: no rim at all
+        assert d.covered.all()
+        assert d.n_bulk + d.n_general == 0 and d.n_fix_a == 0 and d.n_fix_b == 0
+    # every table handed to the library is a real (non-empty) array, also
+    # for an empty rim
+    for a in (d.deep_ids, d.deep_walls, d.tile_list, d.bulk_masks, d.fix_a,
+              d.fix_b) + ((d.wall_index,) if d.wall_index is not None else ()):
+        assert a.size >= 1
     # rim tiles: exactly the tiles not inside a deep tile
     listed = np.zeros(t.nti * ntj, dtype=int)
     for tid in d.tile_list[:d.n_bulk + d.n_general]:
