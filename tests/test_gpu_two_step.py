@@ -32,6 +32,23 @@ def _forced_channel(ny=300):
     return sim
 
 
+def _periodic_box(nx=60, ny=300):
+    """Periodic in x and y, Guo-forced: deep tiles take the periodic rows,
+    the periodic columns stay in the rim."""
+    box = cases.poiseuille_channel(nx, ny, model="BGK", forcing="guo_linear")
+    box.boundary_dict = {
+        "options": {},
+        "x": {"wall": False,
+              "segments": [[[0, 0], [0, ny - 1]], [[nx - 1, 0], [nx - 1, ny - 1]]],
+              "fluid": {"type": "periodic"}},
+        "y": {"wall": False,
+              "segments": [[[0, 0], [nx - 1, 0]], [[0, ny - 1], [nx - 1, ny - 1]]],
+              "fluid": {"type": "periodic"}},
+    }
+    box.forcing_dict["gravity"] = [1e-5, 2e-5]
+    return box
+
+
 CASES = {
     "cylinder": lambda: cases.cylinder(120, 300, diameter=20, xc=40),
     "cavity": lambda: cases.lid_driven_cavity(100, 300),
@@ -48,6 +65,7 @@ CASES = {
     # and rows a multiple of the tile height
     "channel_rows_multiple": lambda: cases.poiseuille_channel(
         72, 256, model="BGK", forcing="guo_linear"),
+    "periodic_box": _periodic_box,
 }
 
 

@@ -185,7 +185,22 @@ DEEP_CASES = {
     "channel": lambda: cases.poiseuille_channel(80, 256, model="BGK"),
     "channel_rows_multiple": lambda: cases.poiseuille_channel(72, 192),
     "channel_rows_odd": lambda: cases.poiseuille_channel(77, 130),
+    "periodic_box": lambda: _periodic_box(),
 }
+
+
+def _periodic_box(nx=60, ny=300):
+    box = cases.poiseuille_channel(nx, ny, model="BGK", forcing="guo_linear")
+    box.boundary_dict = {
+        "options": {},
+        "x": {"wall": False,
+              "segments": [[[0, 0], [0, ny - 1]], [[nx - 1, 0], [nx - 1, ny - 1]]],
+              "fluid": {"type": "periodic"}},
+        "y": {"wall": False,
+              "segments": [[[0, 0], [nx - 1, 0]], [[0, ny - 1], [nx - 1, ny - 1]]],
+              "fluid": {"type": "periodic"}},
+    }
+    return box
 
 
 @pytest.mark.parametrize("name", sorted(DEEP_CASES))
