@@ -180,8 +180,8 @@ __device__ __forceinline__ void deep_wall_global(const d2q9_lattice& L,
 }
 
 // Two workgroups (32 waves) per CU need 8 waves per SIMD, i.e. at most 64
-// VGPRs, and every form is held to that: BGK takes 42-62 VGPRs, the
-// reduced MRT 40-56.
+// VGPRs, and every form is held to that: BGK takes 42-61 VGPRs, the
+// reduced MRT 38-54.
 //
 // Whether the wave index is taken as wave-uniform (readfirstlane), so that
 // the rows' addresses are formed once per wave in scalar registers instead
