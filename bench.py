@@ -558,8 +558,9 @@ def run_legs(args, adj_leg):
     tile = tuple(int(v) for v in args.tile.split("x"))
     if args.profile_run and args.workload == "batched":
         bat = batched_form(args.steps, args.warmup, per_lattice_steps=0)
+        bw = copy_bandwidth(1 << 30, reps=5)
         log(f"profile run batched: {bat['us_per_step']:.1f} us/step "
-            f"{bat['mlups']:.0f} MLUPS config {bat['config']}")
+            f"{bat['mlups']:.0f} MLUPS config {bat['config']}, copy {bw:.0f} GB/s")
         return None
     sim = make_workload(args.workload, args.nx, args.ny,
                         args.steps + args.warmup)
