@@ -335,7 +335,10 @@ class CaseTables:
         cells (block-aligned, rows a multiple of the tile rows) whose cells
         and neighbours one cell deep are all plain fluid -- no boundary
         rule, solid neighbour, stored moment or periodic edge, and inside
-        the lattice -- except that the first or last column of the lattice
+        the lattice, where the rows past the ends of a lattice periodic in
+        x with at least 2 ``rows`` rows are their periodic images (and its
+        last row of tiles may be moved up, see below) -- except that the
+        first or last column of the lattice
         may be a straight zero-gradient or bounce-back wall swept in the
         tile (wall rows of sides 0-3, one kind along the tile's rows; then
         the ghost column beside it is the one cell of the grown region
