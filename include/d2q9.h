@@ -228,7 +228,11 @@ int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
  * ceil(ny / tile_j)) whose cells and whose neighbours one cell deep are plain
  * fluid (flag D2Q9_CELL_FLUID only, no periodic edge); the caller
  * guarantees it (a tile whose grown region leaves the lattice is skipped,
- * so a bad list cannot make an access out of bounds).  deep_walls (null:
+ * so a bad list cannot make an access out of bounds).  In a lattice
+ * periodic in x with at least 2 d2q9_deep_rows() rows, rows past the first
+ * and last are read from their periodic images, so tiles may reach them,
+ * and a tile that would pass row nx is moved up to end at row nx (cells of
+ * two tiles are written twice with the same bits).  deep_walls (null:
  * none) gives per deep tile the straight wall its grown region may hold at
  * j = 1 (bits 0-1) and at j = ny (bits 2-3): 1 zero-gradient, 2
  * bounce-back, every cell of that column in the grown rows a wall cell of
@@ -239,8 +243,10 @@ int d2q9_fused_step(const d2q9_lattice* L, const void* f_src, void* f_dst,
  * L_rim_b covers every cell outside the deep tiles, L_rim_a those cells and
  * also every cell they pull from.  The rim runs as two single steps through
  * f_tmp / mom_tmp (scratch of the sizes of f_src / mom_src) on the
- * auxiliary stream, forked and joined as for d2q9_fused_step.  All three
- * population buffers and all three moment buffers must differ. */
+ * auxiliary stream, forked and joined as for d2q9_fused_step; when both rim
+ * descriptors list no tile and no fix-up cell, the deep tiles alone run on
+ * `stream`, with no fork and join.  All three population buffers and all
+ * three moment buffers must differ. */
 int d2q9_fused_step2(const d2q9_lattice* L, const d2q9_lattice* L_rim_a,
                      const d2q9_lattice* L_rim_b, const int32_t* deep_tiles,
                      const uint8_t* deep_walls, int32_t n_deep, const void* f_src,
