@@ -119,3 +119,26 @@ def test_fp32_cylinder_mean_flow_has_no_momentum_rounding_bias():
     print(f"[precision] FP32 BGK cylinder {nx}x{ny}, {steps} steps: u rel L2 {err:.2e}",
           flush=True)
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("storage,bar", [("f16dev", 1e-3), ("bf16dev", 7e-3)])
+def test_16bit_channel_error_floor(storage, bar):
+    """The 16-bit storage forms' floor: stochastic rounding of the stored
+    deviations is unbiased noise, about 4e-4 (FP16) and 2.7e-3 (BF16) of u
+    on a 256x256 MRT + Guo channel after 20,000 single steps.  The bars
+    (2.5x those) catch a bias coming back: a constant rounding factor in
+    the equilibrium had put FP16 at 2.4e-3 on the cylinder."""
+    from pylabolt_amd.solvers.fluidLB import Solver
+    n, steps = 256, 20000
+    u = {}
+    for st in ("f64", storage):
+        s = Solver(simulation=cases.poiseuille_channel(n, n), storage=st,
+                   verbose=False, graph_steps=50)
+        s.set_backend()
+        s.advance(steps)
+        _, vel = s.engine.moments_host()
+        u[st] = vel.reshape(n + 2, n + 2, 2)[1:-1, 1:-1].copy()
+    err = float(np.linalg.norm(u[storage] - u["f64"]) / np.linalg.norm(u["f64"]))
+    print(f"[precision] {storage} MRT+Guo channel {n}x{n}, {steps} steps: u rel L2 {err:.2e}",
+          flush=True)
+    assert err <= bar
